@@ -1,0 +1,156 @@
+// ORACLE — test infrastructure only (see sbp.hpp). C-ABI of the CPU restatement,
+// built by oracle/Makefile into oracle/_build/libcpu_ref.so. Every function has a
+// float (_f) and a double (_d) variant; the double build is the FP64 oracle.
+#include "wave.hpp"
+
+extern "C" {
+
+// Injector tables (the arrays of the reference's injector: src/include/injector.hpp:8-16).
+typedef struct {
+    int npts, kind;  // kind 0 delta_m3, 1 ddelta_m3
+    const int *xi, *yi, *zi;
+    const void *xw, *yw, *zw;
+} orc_inj;
+
+// Propagation parameters (subset of the reference's param struct: src/include/param.hpp:8-71).
+typedef struct {
+    int nt, sub;
+    double dt;
+    int bc[6], taper[6];
+    double taper_strength, free_surface_stiffness;
+} orc_par;
+}
+
+namespace {
+
+template <class T>
+orc::Injector<T> to_inj(const orc_inj* s) {
+    orc::Injector<T> I;
+    I.npts = s->npts;
+    I.kind = s->kind;
+    const int nw = I.nw();
+    I.xi.assign(s->xi, s->xi + 3 * s->npts);
+    I.yi.assign(s->yi, s->yi + 3 * s->npts);
+    I.zi.assign(s->zi, s->zi + 3 * s->npts);
+    const T* xw = (const T*)s->xw;
+    const T* yw = (const T*)s->yw;
+    const T* zw = (const T*)s->zw;
+    I.xw.assign(xw, xw + 2 * nw * s->npts);
+    I.yw.assign(yw, yw + 2 * nw * s->npts);
+    I.zw.assign(zw, zw + 2 * nw * s->npts);
+    return I;
+}
+
+template <class T>
+orc::PropParams<T> to_par(const orc_par* p) {
+    orc::PropParams<T> q;
+    q.nt = p->nt;
+    q.sub = p->sub;
+    q.dt = (T)p->dt;
+    for (int i = 0; i < 6; i++) { q.bc[i] = p->bc[i]; q.taper[i] = p->taper[i]; }
+    q.taper_strength = (T)p->taper_strength;
+    q.free_surface_stiffness = (T)p->free_surface_stiffness;
+    return q;
+}
+
+template <class T>
+int propagate_t(int physics, int adj, const T* model, const T* src, T* rcv, const orc_inj* inj, const orc_inj* ext,
+                T* grad, T* snaps, const orc_par* par, int nx, int ny, int nz, T dx, T dy, T dz) {
+    try {
+        orc::Grid g{nx, ny, nz};
+        auto I = to_inj<T>(inj);
+        auto E = to_inj<T>(ext);
+        auto P = to_par<T>(par);
+        if (physics == orc::PHYS_ACOUSTIC)
+            orc::propagate_acoustic<T>(adj, model, src, rcv, I, E, grad, snaps, P, g, dx, dy, dz);
+        else
+            orc::propagate_elastic<T>(physics == orc::PHYS_VTI, adj, model, src, rcv, I, E, grad, snaps, P, g, dx, dy, dz);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+}  // namespace
+
+#define ORC_EXPORTS(T, S)                                                                                              \
+    int orc_d1_##S(int axis, int add, const T* in, T* out, int nx, int ny, int nz, T h, const T* par) {               \
+        orc::d1<T>(axis, add, in, out, orc::Grid{nx, ny, nz}, h, par);                                                 \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_d2_##S(int axis, int add, const T* in, T* out, const T* b, int nx, int ny, int nz, T h, T a) {            \
+        try { orc::d2<T>(axis, add, in, out, b, orc::Grid{nx, ny, nz}, h, a); } catch (...) { return -1; }            \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_apply_h_##S(int axis, int inv, int add, const T* in, T* out, int nx, int ny, int nz, T h) {               \
+        orc::apply_h<T>(axis, inv, add, in, out, orc::Grid{nx, ny, nz}, h);                                            \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_taper_##S(T* u, int ncomp, int nx, int ny, int nz, const int* tp, T a) {                                  \
+        orc::taper<T>(u, ncomp, orc::Grid{nx, ny, nz}, tp, a);                                                         \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_asat_##S(int face, int bc, const T* curr, const T* prev, T* next, const T* model, int nx, int ny, int nz,  \
+                     T dx, T dy, T dz, T dt, T alpha) {                                                                \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        orc::asat_face<T>(face, bc, curr, prev, next, model, model + g.n(), g, dx, dy, dz, dt, alpha);                \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_ascale_##S(T* next, const T* model, int nx, int ny, int nz, T dx, T dy, T dz, T dt, const int* bc) {      \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        orc::ascale<T>(next, model, model + g.n(), g, dx, dy, dz, dt, bc);                                            \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_esat_##S(int face, int bc, int vti, const T* curr, const T* prev, T* next, const T* model, int nx,         \
+                     int ny, int nz, T dx, T dy, T dz, T dt) {                                                         \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        const size_t n = g.n();                                                                                        \
+        orc::EMod<T> M;                                                                                                \
+        M.vti = vti;                                                                                                   \
+        for (int k = 0; k < 6; k++) M.m[k] = (vti || k < 3) ? model + k * n : nullptr;                                \
+        const T* c[3] = {curr, curr + n, curr + 2 * n};                                                                \
+        const T* p[3] = {prev, prev + n, prev + 2 * n};                                                                \
+        T* x[3] = {next, next + n, next + 2 * n};                                                                      \
+        if (bc == 1 || bc == 2) orc::esat_face<T>(face, bc, c, p, x, M, g, dx, dy, dz, dt);                          \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_escale_##S(int vti, T* next, const T* model, int nx, int ny, int nz, T dx, T dy, T dz, T dt,              \
+                       const int* bc) {                                                                                \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        const size_t n = g.n();                                                                                        \
+        orc::EMod<T> M;                                                                                                \
+        M.vti = vti;                                                                                                   \
+        for (int k = 0; k < 6; k++) M.m[k] = (vti || k < 3) ? model + k * n : nullptr;                                \
+        T* x[3] = {next, next + n, next + 2 * n};                                                                      \
+        orc::escale<T>(x, M, g, dx, dy, dz, dt, bc);                                                                   \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_injector_##S(int kind, const double* loc, int npts, int nx, int ny, int nz, double dx, double dy,         \
+                         double dz, double ox, double oy, double oz, int* xi, int* yi, int* zi, T* xw, T* yw, T* zw) { \
+        auto I = orc::make_injector<T>(kind, loc, npts, orc::Grid{nx, ny, nz}, dx, dy, dz, ox, oy, oz);                \
+        std::copy(I.xi.begin(), I.xi.end(), xi);                                                                       \
+        std::copy(I.yi.begin(), I.yi.end(), yi);                                                                       \
+        std::copy(I.zi.begin(), I.zi.end(), zi);                                                                       \
+        std::copy(I.xw.begin(), I.xw.end(), xw);                                                                       \
+        std::copy(I.yw.begin(), I.yw.end(), yw);                                                                       \
+        std::copy(I.zw.begin(), I.zw.end(), zw);                                                                       \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_inject_##S(const orc_inj* inj, const T* src, int nt, int comp, int it, T* out, int nx, int ny, int nz) {  \
+        orc::inject<T>(to_inj<T>(inj), src, nt, comp, it, out, orc::Grid{nx, ny, nz});                                 \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_extract_##S(const orc_inj* inj, const T* u, int nt, int comp, int it, T* rcv, int nx, int ny, int nz) {   \
+        orc::extract<T>(to_inj<T>(inj), u, nt, comp, it, rcv, orc::Grid{nx, ny, nz});                                  \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_propagate_##S(int physics, int adj, const T* model, const T* src, T* rcv, const orc_inj* inj,             \
+                          const orc_inj* ext, T* grad, T* snaps, const orc_par* par, int nx, int ny, int nz, T dx,     \
+                          T dy, T dz) {                                                                                \
+        return propagate_t<T>(physics, adj, model, src, rcv, inj, ext, grad, snaps, par, nx, ny, nz, dx, dy, dz);      \
+    }
+
+extern "C" {
+ORC_EXPORTS(float, f)
+ORC_EXPORTS(double, d)
+}

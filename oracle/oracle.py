@@ -1,0 +1,198 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings of the CPU restatement (oracle/_build/libcpu_ref.so) and, where it
+was built, of the reference's own operators (oracle/_ref/libref_ops_{f,d}.so).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this,
+as the checker; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CPU_REF = os.path.join(HERE, "_build", "libcpu_ref.so")
+REF_OPS = {"f": os.path.join(HERE, "_ref", "libref_ops_f.so"), "d": os.path.join(HERE, "_ref", "libref_ops_d.so")}
+
+PHYS = {"acoustic": 0, "elastic": 1, "vti": 2}
+FACES = ("top", "bottom", "left", "right", "front", "back")
+
+
+class Inj(C.Structure):
+    _fields_ = [("npts", C.c_int), ("kind", C.c_int),
+                ("xi", C.c_void_p), ("yi", C.c_void_p), ("zi", C.c_void_p),
+                ("xw", C.c_void_p), ("yw", C.c_void_p), ("zw", C.c_void_p)]
+
+
+class Par(C.Structure):
+    _fields_ = [("nt", C.c_int), ("sub", C.c_int), ("dt", C.c_double),
+                ("bc", C.c_int * 6), ("taper", C.c_int * 6),
+                ("taper_strength", C.c_double), ("free_surface_stiffness", C.c_double)]
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _dt(s):
+    return np.float32 if s == "f" else np.float64
+
+
+_lib = None
+_refs = {}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(CPU_REF):
+            raise RuntimeError(f"oracle library missing: {CPU_REF} (run `make -C oracle`)")
+        _lib = C.CDLL(CPU_REF)
+    return _lib
+
+
+def ref_available(s="f"):
+    return os.path.exists(REF_OPS[s])
+
+
+def ref(s="f"):
+    if s not in _refs:
+        if not ref_available(s):
+            raise RuntimeError(f"reference operator library missing: {REF_OPS[s]} (run `make -C oracle ref`)")
+        _refs[s] = C.CDLL(REF_OPS[s])
+    return _refs[s]
+
+
+class Ops:
+    """Operator-level calls with the same argument meaning for the oracle
+    (`prefix='orc'`) and the compiled reference (`prefix='ref'`)."""
+
+    def __init__(self, prefix="orc", s="f"):
+        self.s = s
+        self.T = _dt(s)
+        self.cT = C.c_float if s == "f" else C.c_double
+        self.L = lib() if prefix == "orc" else ref(s)
+        self.p = prefix
+
+    def _f(self, name):
+        return getattr(self.L, f"{self.p}_{name}_{self.s}")
+
+    def _chk(self, *arrs):
+        for a in arrs:
+            if a is not None:
+                assert a.dtype == self.T and a.flags.c_contiguous
+
+    def d1(self, axis, inp, out, h, par=None, add=False):
+        ny, nx, nz = inp.shape[-3:]
+        self._chk(inp, out, par)
+        self._f("d1")(axis, int(add), _ptr(inp), _ptr(out), nx, ny, nz, self.cT(h), _ptr(par))
+        return out
+
+    def d2(self, axis, inp, out, b, h, a=1.0, add=False):
+        ny, nx, nz = inp.shape[-3:]
+        self._chk(inp, out, b)
+        self._f("d2")(axis, int(add), _ptr(inp), _ptr(out), _ptr(b), nx, ny, nz, self.cT(h), self.cT(a))
+        return out
+
+    def apply_h(self, axis, inp, out, h, inv=False, add=False):
+        ny, nx, nz = inp.shape[-3:]
+        self._chk(inp, out)
+        self._f("apply_h")(axis, int(inv), int(add), _ptr(inp), _ptr(out), nx, ny, nz, self.cT(h))
+        return out
+
+    def taper(self, u, tp, strength):
+        ny, nx, nz = u.shape[-3:]
+        ncomp = u.size // (nx * ny * nz)
+        self._chk(u)
+        tpa = (C.c_int * 6)(*tp)
+        self._f("taper")(_ptr(u), ncomp, nx, ny, nz, tpa, self.cT(strength))
+        return u
+
+    def asat(self, face, bc, curr, prev, nxt, model, d, dt, alpha):
+        ny, nx, nz = curr.shape[-3:]
+        self._chk(curr, prev, nxt, model)
+        self._f("asat")(face, bc, _ptr(curr), _ptr(prev), _ptr(nxt), _ptr(model), nx, ny, nz,
+                        self.cT(d[0]), self.cT(d[1]), self.cT(d[2]), self.cT(dt), self.cT(alpha))
+        return nxt
+
+    def ascale(self, nxt, model, d, dt, bc):
+        ny, nx, nz = nxt.shape[-3:]
+        self._chk(nxt, model)
+        self._f("ascale")(_ptr(nxt), _ptr(model), nx, ny, nz, self.cT(d[0]), self.cT(d[1]), self.cT(d[2]),
+                          self.cT(dt), (C.c_int * 6)(*bc))
+        return nxt
+
+    def esat(self, face, bc, vti, curr, prev, nxt, model, d, dt):
+        ny, nx, nz = curr.shape[-3:]
+        self._chk(curr, prev, nxt, model)
+        self._f("esat")(face, bc, int(vti), _ptr(curr), _ptr(prev), _ptr(nxt), _ptr(model), nx, ny, nz,
+                        self.cT(d[0]), self.cT(d[1]), self.cT(d[2]), self.cT(dt))
+        return nxt
+
+    def escale(self, vti, nxt, model, d, dt, bc):
+        ny, nx, nz = nxt.shape[-3:]
+        self._chk(nxt, model)
+        self._f("escale")(int(vti), _ptr(nxt), _ptr(model), nx, ny, nz, self.cT(d[0]), self.cT(d[1]),
+                          self.cT(d[2]), self.cT(dt), (C.c_int * 6)(*bc))
+        return nxt
+
+
+class Injector:
+    """Index/weight tables of a point injector (kind 0 delta_m3, 1 ddelta_m3)."""
+
+    def __init__(self, kind, loc, shape, d, o, s="f"):
+        ny, nx, nz = shape
+        loc = np.ascontiguousarray(np.asarray(loc, dtype=np.float64).reshape(-1, 3))
+        self.npts = loc.shape[0]
+        self.kind = kind
+        nw = 6 if kind else 3
+        T = _dt(s)
+        self.xi = np.zeros((self.npts, 3), np.int32)
+        self.yi = np.zeros_like(self.xi)
+        self.zi = np.zeros_like(self.xi)
+        self.xw = np.zeros((self.npts, 2, nw), T)
+        self.yw = np.zeros_like(self.xw)
+        self.zw = np.zeros_like(self.xw)
+        getattr(lib(), f"orc_injector_{s}")(kind, _ptr(loc), self.npts, nx, ny, nz,
+                                            C.c_double(d[0]), C.c_double(d[1]), C.c_double(d[2]),
+                                            C.c_double(o[0]), C.c_double(o[1]), C.c_double(o[2]),
+                                            _ptr(self.xi), _ptr(self.yi), _ptr(self.zi),
+                                            _ptr(self.xw), _ptr(self.yw), _ptr(self.zw))
+
+    def cstruct(self):
+        return Inj(self.npts, self.kind, self.xi.ctypes.data, self.yi.ctypes.data, self.zi.ctypes.data,
+                   self.xw.ctypes.data, self.yw.ctypes.data, self.zw.ctypes.data)
+
+
+def make_par(nt, sub, dt, bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6, taper_strength=0.05, free_surface_stiffness=1.05):
+    return Par(nt, sub, dt, (C.c_int * 6)(*bc), (C.c_int * 6)(*taper), taper_strength, free_surface_stiffness)
+
+
+def n_snap_slots(nt, sub):
+    return 1 + nt // sub if sub > 0 else 0
+
+
+def propagate(physics, model, src, inj, ext, par, d, adj=False, grad=None, snaps=None, s="f"):
+    """Run the oracle's propagate. model: converted parameters [np][ny][nx][nz].
+    src: [ns][nsrc][nt]. Returns rcv [nrc][nr][nt] (forward) or None (adjoint; grad updated)."""
+    T = _dt(s)
+    cT = C.c_float if s == "f" else C.c_double
+    ph = PHYS[physics] if isinstance(physics, str) else physics
+    ny, nx, nz = model.shape[-3:]
+    ncomp = 1 if ph == 0 else 3
+    model = np.ascontiguousarray(model, dtype=T)
+    src = np.ascontiguousarray(src, dtype=T)
+    rcv = None
+    if not adj:
+        rcv = np.zeros((ncomp, ext.npts, par.nt), T)
+    if snaps is None and par.sub > 0:
+        snaps = np.zeros((n_snap_slots(par.nt, par.sub), ncomp, ny, nx, nz), T)
+    I, E = inj.cstruct(), ext.cstruct()
+    rc = getattr(lib(), f"orc_propagate_{s}")(ph, int(adj), _ptr(model), _ptr(src), _ptr(rcv), C.byref(I), C.byref(E),
+                                              _ptr(grad), _ptr(snaps), C.byref(par), nx, ny, nz,
+                                              cT(d[0]), cT(d[1]), cT(d[2]))
+    if rc != 0:
+        raise RuntimeError("oracle propagate failed")
+    return rcv, snaps
