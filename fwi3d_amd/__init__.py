@@ -1,0 +1,185 @@
+"""fwi3d_amd — MI355X-native time-domain wave propagation and adjoint gradients.
+
+Python bindings (ctypes) of the C ABI declared in include/fwi3d_hip.h and
+implemented by fwi3d_amd/libfwi3d_amd.so (HIP, gfx950). The library is the product;
+this module only marshals arguments. Importing works without a GPU; creating an
+Engine needs one, and every call fails loudly (RuntimeError) if the native library
+is missing or returns an error — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfwi3d_amd.so")
+
+ACOUSTIC, ELASTIC, VTI = 0, 1, 2
+PHYSICS = {"acoustic": ACOUSTIC, "elastic": ELASTIC, "vti": VTI}
+SNAP = {"f32": 0, "bf16": 1, "f16": 2}
+
+# Exported symbols of include/fwi3d_hip.h (checked by the CPU test suite).
+EXPORTS = (
+    "fwi3d_create", "fwi3d_destroy", "fwi3d_last_error", "fwi3d_injector_tables",
+    "fwi3d_propagate", "fwi3d_propagate_device", "fwi3d_profile", "fwi3d_profile_read",
+)
+
+
+class Param(C.Structure):
+    """fwi3d_param (include/fwi3d_hip.h)."""
+    _fields_ = [("nt", C.c_int), ("sub", C.c_int), ("dt", C.c_float),
+                ("bc", C.c_int * 6), ("taper", C.c_int * 6),
+                ("taper_strength", C.c_float), ("free_surface_stiffness", C.c_float),
+                ("snap_precision", C.c_int)]
+
+
+class InjectorStruct(C.Structure):
+    """fwi3d_injector (include/fwi3d_hip.h)."""
+    _fields_ = [("npts", C.c_int), ("kind", C.c_int),
+                ("xind", C.c_void_p), ("yind", C.c_void_p), ("zind", C.c_void_p),
+                ("xw", C.c_void_p), ("yw", C.c_void_p), ("zw", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"fwi3d_amd native library not built: {LIB_PATH} (run python -m fwi3d_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.fwi3d_last_error.restype = C.c_char_p
+        L.fwi3d_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+        L.fwi3d_destroy.argtypes = [C.c_void_p]
+        L.fwi3d_last_error.argtypes = [C.c_void_p]
+        L.fwi3d_profile.argtypes = [C.c_void_p, C.c_int]
+        L.fwi3d_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def make_param(nt, dt, sub=0, bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6, taper_strength=0.05,
+               free_surface_stiffness=1.05, snap_precision="f32"):
+    sp = SNAP[snap_precision] if isinstance(snap_precision, str) else int(snap_precision)
+    return Param(int(nt), int(sub), float(dt), (C.c_int * 6)(*bc), (C.c_int * 6)(*taper),
+                 float(taper_strength), float(free_surface_stiffness), sp)
+
+
+class Injector:
+    """Injector tables for point sources/receivers (delta_m3: kind 0, ddelta_m3: kind 1),
+    computed by the library (fwi3d_injector_tables)."""
+
+    def __init__(self, kind, loc, shape, d, o=(0.0, 0.0, 0.0)):
+        ny, nx, nz = shape
+        loc = np.ascontiguousarray(np.asarray(loc, dtype=np.float32).reshape(-1, 3))
+        self.npts = int(loc.shape[0])
+        self.kind = int(kind)
+        nw = 6 if kind else 3
+        self.xind = np.zeros((self.npts, 3), np.int32)
+        self.yind = np.zeros_like(self.xind)
+        self.zind = np.zeros_like(self.xind)
+        self.xw = np.zeros((self.npts, 2, nw), np.float32)
+        self.yw = np.zeros_like(self.xw)
+        self.zw = np.zeros_like(self.xw)
+        rc = lib().fwi3d_injector_tables(
+            self.kind, C.c_void_p(loc.ctypes.data), self.npts, nx, ny, nz,
+            C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]),
+            C.c_float(o[0]), C.c_float(o[1]), C.c_float(o[2]),
+            C.c_void_p(self.xind.ctypes.data), C.c_void_p(self.yind.ctypes.data), C.c_void_p(self.zind.ctypes.data),
+            C.c_void_p(self.xw.ctypes.data), C.c_void_p(self.yw.ctypes.data), C.c_void_p(self.zw.ctypes.data))
+        if rc != 0:
+            raise ValueError("point outside the grid or invalid injector arguments")
+
+    def cstruct(self):
+        return InjectorStruct(self.npts, self.kind, self.xind.ctypes.data, self.yind.ctypes.data,
+                              self.zind.ctypes.data, self.xw.ctypes.data, self.yw.ctypes.data, self.zw.ctypes.data)
+
+
+def _vp(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def n_model(physics):
+    return {ACOUSTIC: 2, ELASTIC: 3, VTI: 6}[physics]
+
+
+def n_comp(physics):
+    return 1 if physics == ACOUSTIC else 3
+
+
+def n_grad(physics):
+    return 2 if physics == ACOUSTIC else 3
+
+
+class Engine:
+    """One device context (fwi3d_ctx): a HIP stream plus the device-resident state of
+    an operator instance (wavefields, snapshots kept from forward to adjoint)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().fwi3d_create(C.byref(self._h), int(device))
+        if rc != 0:
+            raise RuntimeError(f"fwi3d_create failed ({rc})")
+
+    def close(self):
+        if self._h:
+            lib().fwi3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().fwi3d_last_error(self._h)
+            raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def propagate(self, physics, model, src, inj, ext, par, d, adj=False, grad=None):
+        """Host-array propagate (fwi3d_propagate). model [np][ny][nx][nz] converted
+        parameters; src [ns][nsrc][nt]; returns rcv [nc][nrcv][nt] for a forward call.
+        grad (float32 [ng][ny][nx][nz]) is accumulated in place for an adjoint call."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        model = np.ascontiguousarray(model, dtype=np.float32)
+        src = np.ascontiguousarray(src, dtype=np.float32)
+        ny, nx, nz = model.shape[-3:]
+        rcv = None if adj else np.zeros((n_comp(ph), ext.npts, par.nt), np.float32)
+        if grad is not None:
+            assert grad.dtype == np.float32 and grad.flags.c_contiguous
+        I = inj.cstruct()
+        E = ext.cstruct() if ext is not None else InjectorStruct(0, 0, None, None, None, None, None, None)
+        rc = lib().fwi3d_propagate(self._h, ph, int(bool(adj)), _vp(model), _vp(src), _vp(rcv), C.byref(I),
+                                   C.byref(E), _vp(grad), C.byref(par), nx, ny, nz,
+                                   C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
+        self._check(rc, "fwi3d_propagate")
+        return rcv
+
+    def propagate_device(self, physics, model_ptr, src_ptr, rcv_ptr, inj, ext, par, shape, d, adj=False,
+                         grad64_ptr=None):
+        """Device-pointer propagate (fwi3d_propagate_device): pointers are integers
+        (e.g. torch.Tensor.data_ptr()) to HBM-resident arrays in reference layouts."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        I = inj.cstruct()
+        E = ext.cstruct() if ext is not None else InjectorStruct(0, 0, None, None, None, None, None, None)
+        rc = lib().fwi3d_propagate_device(self._h, ph, int(bool(adj)), C.c_void_p(model_ptr), C.c_void_p(src_ptr),
+                                          C.c_void_p(rcv_ptr) if rcv_ptr else None, C.byref(I), C.byref(E),
+                                          C.c_void_p(grad64_ptr) if grad64_ptr else None, C.byref(par), nx, ny, nz,
+                                          C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
+        self._check(rc, "fwi3d_propagate_device")
+
+    def profile(self, enable=True):
+        self._check(lib().fwi3d_profile(self._h, int(bool(enable))), "fwi3d_profile")
+
+    def profile_read(self, kernel=0):
+        """(milliseconds, launches) of kernel class 0 (time step) or 1 (flux pre-pass)."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._check(lib().fwi3d_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)), "fwi3d_profile_read")
+        return ms.value, n.value

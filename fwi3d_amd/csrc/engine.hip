@@ -1,0 +1,671 @@
+// Engine: context, device-resident state and the time loop of propagate(), behind
+// the C ABI of include/fwi3d_hip.h. All kernels are in this translation unit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fwi3d_hip.h"
+#include "acoustic.hpp"
+#include "common.hpp"
+#include "elastic.hpp"
+#include "sbp_tables.hpp"
+
+using namespace fwi3d;
+
+namespace {
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIP_OK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            throw Err{FWI3D_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)};             \
+    } while (0)
+
+#define REQUIRE(cond, text)                                    \
+    do {                                                       \
+        if (!(cond)) throw Err{FWI3D_EINVAL, std::string(text)}; \
+    } while (0)
+
+// Growable device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t nb) {
+        if (nb <= bytes) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) throw Err{FWI3D_ENOMEM, "hipMalloc(" + std::to_string(nb) + " bytes) failed"};
+        bytes = nb;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+bool g_tables_loaded[64] = {false};
+
+void load_tables(int dev) {
+    if (dev >= 0 && dev < 64 && g_tables_loaded[dev]) return;
+    float d2[6][8][8] = {};
+    for (const auto& e : kD2Closure) d2[e.r][e.j][e.k] = (float)((double)e.num / (double)e.den);
+    float d1[4][6];
+    for (int r = 0; r < 4; r++)
+        for (int j = 0; j < 6; j++) d1[r][j] = (float)kD1Closure[r][j];
+    float s[4], h[4];
+    for (int k = 0; k < 4; k++) { s[k] = (float)kSatS[k]; h[k] = (float)kHnorm[k]; }
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_d2b), d2, sizeof(d2)));
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_d1b), d1, sizeof(d1)));
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sat_s), s, sizeof(s)));
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_hnorm), h, sizeof(h)));
+    if (dev >= 0 && dev < 64) g_tables_loaded[dev] = true;
+}
+
+}  // namespace
+
+struct fwi3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;
+    // snapshot-store state (the per-instance state of the reference operator)
+    int snap_slots = 0, snap_ncomp = 0, snap_nt = 0, snap_sub = 0, snap_physics = -1;
+    long long snap_n = 0;
+    bool snap_valid = false;
+    int snap_dims[3] = {0, 0, 0};
+    // profiling: [0] time-step kernel, [1] flux pre-pass (elastic/VTI)
+    bool prof = false;
+    double prof_ms[2] = {0, 0};
+    long long prof_n[2] = {0, 0};
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;
+};
+
+namespace {
+
+Geom make_geom(int nx, int ny, int nz, float dx, float dy, float dz) {
+    Geom g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.pz = (nz + 63) / 64 * 64;
+    g.sx = g.pz;
+    g.sy = (long long)nx * g.pz;
+    g.n = (long long)ny * g.sy;
+    g.dx = dx; g.dy = dy; g.dz = dz;
+    return g;
+}
+
+// Squared cos taper factor per index along one axis, in the float arithmetic of the
+// reference (val = (float)cos(double), then val*val): spatial_operators.cpp:408-506.
+void taper_table(std::vector<float>& t, int n, int lo, int hi, float strength) {
+    t.assign(n, 1.0f);
+    const double a = (double)strength;
+    for (int i = 0; i < lo && i < n; i++) {
+        float v = (float)std::cos(a * 0.5 * M_PI * (double)(i + 1 - lo) / (double)(0 - lo));
+        t[i] = v * v;
+    }
+    for (int i = std::max(0, n - hi); i < n; i++) {
+        float v = (float)std::cos(a * 0.5 * M_PI * (double)(i - (n - hi)) / (double)hi);
+        t[i] = v * v;
+    }
+}
+
+// Copy a model/field set [k][ny][nx][nz] (unpadded, device) into the padded layout.
+void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g) {
+    for (int q = 0; q < k; q++)
+        HIP_OK(hipMemcpy2DAsync(dst + q * g.n, g.pz * sizeof(float), src + (size_t)q * g.nx * g.ny * g.nz,
+                                g.nz * sizeof(float), g.nz * sizeof(float), (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,
+                                c->stream));
+}
+
+struct DevInj {
+    InjTab t;
+};
+
+// Upload injector tables into `buf`: ints then floats.
+InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
+    InjTab t{};
+    t.npts = h->npts;
+    t.kind = h->kind;
+    t.nw = h->kind ? 6 : 3;
+    const size_t ni = (size_t)3 * h->npts, nw = (size_t)2 * t.nw * h->npts;
+    const size_t bytes = 3 * ni * sizeof(int) + 3 * nw * sizeof(float) + 256;
+    buf.ensure(bytes);
+    char* base = buf.as<char>();
+    int* xi = (int*)base;
+    int* yi = xi + ni;
+    int* zi = yi + ni;
+    float* xw = (float*)(zi + ni);
+    float* yw = xw + nw;
+    float* zw = yw + nw;
+    if (h->npts > 0) {
+        HIP_OK(hipMemcpyAsync(xi, h->xind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(yi, h->yind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(zi, h->zind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(xw, h->xw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(yw, h->yw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(zw, h->zw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    t.xi = xi; t.yi = yi; t.zi = zi; t.xw = xw; t.yw = yw; t.zw = zw;
+    return t;
+}
+
+void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char* what) {
+    REQUIRE(h != nullptr, std::string(what) + " injector is null");
+    REQUIRE(h->npts >= 0 && (h->kind == 0 || h->kind == 1), std::string(what) + ": bad npts/kind");
+    for (int i = 0; i < 3 * h->npts; i++) {
+        REQUIRE(h->xind[i] >= 0 && h->xind[i] < nx && h->yind[i] >= 0 && h->yind[i] < ny && h->zind[i] >= 0 &&
+                    h->zind[i] < nz,
+                std::string(what) + " point outside the grid");
+    }
+}
+
+// Event-bracketed launch of a step kernel when profiling.
+template <class F>
+void timed(fwi3d_ctx* c, int kind, F&& launch) {
+    if (!c->prof) {
+        launch();
+        return;
+    }
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, c->stream));
+    launch();
+    HIP_OK(hipEventRecord(b, c->stream));
+    c->ev.push_back(a);
+    c->ev.push_back(b);
+    c->ev_kind.push_back(kind);
+}
+
+void drain_events(fwi3d_ctx* c) {
+    if (c->ev.empty()) return;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < c->ev.size(); i += 2) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        const int k = c->ev_kind[i / 2];
+        c->prof_ms[k] += ms;
+        c->prof_n[k] += 1;
+        (void)hipEventDestroy(c->ev[i]);
+        (void)hipEventDestroy(c->ev[i + 1]);
+    }
+    c->ev.clear();
+    c->ev_kind.clear();
+}
+
+// --------------------------------------------------------------------------------------
+void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const float* d_src, float* d_rcv,
+                        const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
+                        const Geom& g) {
+    const long long n = g.n;
+    c->fields.ensure(3 * n * sizeof(float));
+    c->model.ensure(2 * n * sizeof(float));
+    float* F = c->fields.as<float>();
+    float* M = c->model.as<float>();
+    HIP_OK(hipMemsetAsync(F, 0, 3 * n * sizeof(float), c->stream));
+    HIP_OK(hipMemsetAsync(M, 0, 2 * n * sizeof(float), c->stream));
+    to_padded(c, M, d_model, 2, g);
+    const float* K = M;
+    const float* B = M + n;
+
+    // tapers
+    std::vector<float> tz, tx, ty;
+    taper_table(tz, g.nz, par->taper[0], par->taper[1], par->taper_strength);
+    taper_table(tx, g.nx, par->taper[2], par->taper[3], par->taper_strength);
+    taper_table(ty, g.ny, par->taper[4], par->taper[5], par->taper_strength);
+    c->tapers.ensure((tz.size() + tx.size() + ty.size()) * sizeof(float));
+    float* dtz = c->tapers.as<float>();
+    float* dtx = dtz + tz.size();
+    float* dty = dtx + tx.size();
+    HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, c->stream));
+    Bound Bd;
+    for (int f = 0; f < 6; f++) Bd.bc[f] = par->bc[f];
+    Bd.tz2 = dtz; Bd.tx2 = dtx; Bd.ty2 = dty;
+
+    InjTab I = upload_inj(c, c->injtab, inj);
+    InjTab E{};
+    if (!adj) E = upload_inj(c, c->exttab, ext);
+
+    const int nt = par->nt, sub = par->sub;
+    const float dt = par->dt;
+    const bool fwd_snap = !adj && sub > 0;
+    if (fwd_snap) {
+        const int slots = 1 + nt / sub;
+        c->snaps.ensure((size_t)slots * n * sizeof(float));
+        HIP_OK(hipMemsetAsync(c->snaps.p, 0, (size_t)slots * n * sizeof(float), c->stream));
+        c->snap_slots = slots; c->snap_ncomp = 1; c->snap_n = n; c->snap_nt = nt; c->snap_sub = sub;
+        c->snap_physics = FWI3D_ACOUSTIC; c->snap_dims[0] = g.nx; c->snap_dims[1] = g.ny; c->snap_dims[2] = g.nz;
+        c->snap_valid = false;
+    }
+    if (adj && d_grad && sub > 0) {
+        REQUIRE(c->snap_valid && c->snap_physics == FWI3D_ACOUSTIC && c->snap_nt == nt && c->snap_sub == sub &&
+                    c->snap_dims[0] == g.nx && c->snap_dims[1] == g.ny && c->snap_dims[2] == g.nz,
+                "adjoint propagation needs the snapshots of a matching forward propagation on this context");
+    }
+    float* snaps = c->snaps.as<float>();
+
+    float* prev = F;
+    float* curr = F + n;
+    float* next = F + 2 * n;
+
+    const int nthr = 256;
+    if (I.npts > 0) {
+        int nb = (I.npts * 27 + nthr - 1) / nthr;
+        hipLaunchKernelGGL(acoustic_inject, dim3(nb), dim3(nthr), 0, c->stream, prev, K, B, g, Bd, I, d_src, nt, 0, dt, 1);
+    }
+
+    constexpr int TX = 4;
+    AcuArgs A;
+    A.K = K; A.b = B; A.g = g; A.B = Bd; A.dt = dt; A.dt2 = dt * dt;
+    A.alpha = 0.2508560249f / par->free_surface_stiffness;
+    A.ntz = (g.nz + 63) / 64;
+    A.ntx = (g.nx + TX - 1) / TX;
+    A.ychunk = std::max(4, std::min(g.ny, 16));
+    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
+    const int nblk = A.ntz * A.ntx * A.nyc;
+
+    const long long nunp = (long long)g.nx * g.ny * g.nz;
+    double* gK = d_grad;
+    double* gB = d_grad ? d_grad + nunp : nullptr;
+    auto grad_event = [&](int itl, const float* adjf) {
+        const float sdt = (float)sub * dt;
+        const int last = nt / sub;
+        const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
+        const float* fn = snaps + (size_t)itl * n;
+        const float* fm = itl > 0 ? snaps + (size_t)(itl - 1) * n : fn;
+        // a slot past the store reads as zero (the reference dereferences a null slot there)
+        const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * n : nullptr;
+        dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+        hipLaunchKernelGGL(acoustic_grad, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, K, g, sdt, branch, gK, gB, 0LL);
+    };
+
+    for (int it = 0; it < nt - 1; it++) {
+        if (fwd_snap && it % sub == 0)
+            HIP_OK(hipMemcpyAsync(snaps + (size_t)(it / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        if (!adj && E.npts > 0)
+            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it, 0);
+        if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
+        A.curr = curr; A.prev = prev; A.next = next;
+        A.prev_tapered = it > 0 ? 1 : 0;
+        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_step<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+        if (I.npts > 0) {
+            int nb = (I.npts * 27 + nthr - 1) / nthr;
+            hipLaunchKernelGGL(acoustic_inject, dim3(nb), dim3(nthr), 0, c->stream, next, K, B, g, Bd, I, d_src, nt, it, dt, 0);
+        }
+        float* t = prev; prev = curr; curr = next; next = t;
+    }
+    if (fwd_snap && (nt - 1) % sub == 0)
+        HIP_OK(hipMemcpyAsync(snaps + (size_t)((nt - 1) / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (!adj && E.npts > 0)
+        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1, 0);
+    if (adj && d_grad && sub > 0) grad_event(0, curr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (fwd_snap) c->snap_valid = true;
+}
+
+__global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict__ eps, const float* __restrict__ mu,
+                               long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c11b[i] = 2.f * (1.f + 2.f * eps[i]) * mu[i];
+}
+
+void setup_bound(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g, Bound& Bd) {
+    std::vector<float> tz, tx, ty;
+    taper_table(tz, g.nz, par->taper[0], par->taper[1], par->taper_strength);
+    taper_table(tx, g.nx, par->taper[2], par->taper[3], par->taper_strength);
+    taper_table(ty, g.ny, par->taper[4], par->taper[5], par->taper_strength);
+    c->tapers.ensure((tz.size() + tx.size() + ty.size()) * sizeof(float));
+    float* dtz = c->tapers.as<float>();
+    float* dtx = dtz + tz.size();
+    float* dty = dtx + tx.size();
+    HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, c->stream));
+    for (int f = 0; f < 6; f++) Bd.bc[f] = par->bc[f];
+    Bd.tz2 = dtz; Bd.tx2 = dtx; Bd.ty2 = dty;
+}
+
+void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, const float* d_src, float* d_rcv,
+                       const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
+                       const Geom& g) {
+    const long long n = g.n;
+    const int npar = vti ? 6 : 3;
+    const int nmf = vti ? 7 : 3;  // device model fields (VTI adds c11b)
+    const int nT = vti ? 3 : 1;
+    c->fields.ensure(9 * n * sizeof(float));
+    c->model.ensure(nmf * n * sizeof(float));
+    c->aux.ensure((nT + 6) * n * sizeof(float));
+    float* F = c->fields.as<float>();
+    float* Mf = c->model.as<float>();
+    HIP_OK(hipMemsetAsync(F, 0, 9 * n * sizeof(float), c->stream));
+    HIP_OK(hipMemsetAsync(Mf, 0, nmf * n * sizeof(float), c->stream));
+    to_padded(c, Mf, d_model, npar, g);
+    EModel M{};
+    M.lam = Mf; M.mu = Mf + n; M.rho = Mf + 2 * n;
+    if (vti) {
+        M.c13 = Mf + 3 * n; M.eps = Mf + 4 * n; M.c66 = Mf + 5 * n; M.c11b = Mf + 6 * n;
+        hipLaunchKernelGGL(vti_c11b_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, Mf + 6 * n, M.eps, M.mu, n);
+    }
+    Bound Bd;
+    setup_bound(c, par, g, Bd);
+    InjTab I = upload_inj(c, c->injtab, inj);
+    REQUIRE(adj || I.kind == 1 || true, "");
+    InjTab E{};
+    if (!adj) E = upload_inj(c, c->exttab, ext);
+
+    const int nt = par->nt, sub = par->sub;
+    const float dt = par->dt;
+    const bool fwd_snap = !adj && sub > 0;
+    const long long slot = 3 * n;
+    if (fwd_snap) {
+        const int slots = 1 + nt / sub;
+        c->snaps.ensure((size_t)slots * slot * sizeof(float));
+        HIP_OK(hipMemsetAsync(c->snaps.p, 0, (size_t)slots * slot * sizeof(float), c->stream));
+        c->snap_slots = slots; c->snap_ncomp = 3; c->snap_n = n; c->snap_nt = nt; c->snap_sub = sub;
+        c->snap_physics = vti ? FWI3D_VTI : FWI3D_ELASTIC;
+        c->snap_dims[0] = g.nx; c->snap_dims[1] = g.ny; c->snap_dims[2] = g.nz;
+        c->snap_valid = false;
+    }
+    if (adj && d_grad && sub > 0) {
+        REQUIRE(c->snap_valid && c->snap_physics == (vti ? FWI3D_VTI : FWI3D_ELASTIC) && c->snap_nt == nt &&
+                    c->snap_sub == sub && c->snap_dims[0] == g.nx && c->snap_dims[1] == g.ny && c->snap_dims[2] == g.nz,
+                "adjoint propagation needs the snapshots of a matching forward propagation on this context");
+    }
+    float* snaps = c->snaps.as<float>();
+    float* prev = F;
+    float* curr = F + 3 * n;
+    float* next = F + 6 * n;
+
+    const int nthr = 256;
+    const int nbi = (I.npts * 27 + nthr - 1) / nthr;
+    auto inject = [&](float* out, int it, int mode) {
+        if (I.npts == 0) return;
+        for (int cc = 0; cc < 3; cc++) {
+            if (vti)
+                hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, M, g, Bd, I, d_src, nt, it, dt, cc, mode);
+            else
+                hipLaunchKernelGGL(elastic_inject<false>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, M, g, Bd, I, d_src, nt, it, dt, cc, mode);
+        }
+    };
+    inject(prev, 0, 1);
+
+    constexpr int TX = 4;
+    EArgs A;
+    A.m = M; A.g = g; A.B = Bd; A.dt = dt; A.dt2 = dt * dt; A.aux = c->aux.as<float>();
+    A.ntz = (g.nz + 63) / 64;
+    A.ntx = (g.nx + TX - 1) / TX;
+    A.ychunk = std::max(4, std::min(g.ny, 16));
+    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
+    const int nblk = A.ntz * A.ntx * A.nyc;
+
+    const long long nunp = (long long)g.nx * g.ny * g.nz;
+    auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
+        const float sdt = (float)sub * dt;
+        const int last = nt / sub;
+        const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
+        const float* fn = snaps + (size_t)itl * slot;
+        const float* fm = itl > 0 ? snaps + (size_t)(itl - 1) * slot : fn;
+        const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * slot : nullptr;
+        dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+        if (vti)
+            hipLaunchKernelGGL(elastic_grad<true>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, M, g, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+        else
+            hipLaunchKernelGGL(elastic_grad<false>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, M, g, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+    };
+    auto extract = [&](const float* u, int it) {
+        if (adj || E.npts == 0) return;
+        for (int cc = 0; cc < 3; cc++)
+            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, u + cc * n, g, E, d_rcv, nt, it, cc);
+    };
+
+    for (int it = 0; it < nt - 1; it++) {
+        if (fwd_snap && it % sub == 0)
+            HIP_OK(hipMemcpyAsync(snaps + (size_t)(it / sub) * slot, curr, slot * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        extract(curr, it);
+        if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
+        A.curr = curr; A.prev = prev; A.next = next;
+        A.prev_tapered = it > 0 ? 1 : 0;
+        if (vti) {
+            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+        } else {
+            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+        }
+        inject(next, it, 0);
+        float* t = prev; prev = curr; curr = next; next = t;
+    }
+    if (fwd_snap && (nt - 1) % sub == 0)
+        HIP_OK(hipMemcpyAsync(snaps + (size_t)((nt - 1) / sub) * slot, curr, slot * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    extract(curr, nt - 1);
+    if (adj && d_grad && sub > 0) grad_event(0, curr, prev);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (fwd_snap) c->snap_valid = true;
+}
+
+void validate(int physics, const fwi3d_param* par, int nx, int ny, int nz, float dx, float dy, float dz) {
+    REQUIRE(physics == FWI3D_ACOUSTIC || physics == FWI3D_ELASTIC || physics == FWI3D_VTI, "unknown physics");
+    REQUIRE(par != nullptr, "null parameters");
+    REQUIRE(nx >= 12 && ny >= 12 && nz >= 12, "every grid axis needs at least 12 points (SBP closures)");
+    REQUIRE(dx > 0 && dy > 0 && dz > 0, "grid spacing must be positive");
+    REQUIRE(par->nt >= 2, "nt must be at least 2");
+    REQUIRE(par->dt > 0, "dt must be positive");
+    REQUIRE(par->sub >= 0, "sub must be >= 0");
+    for (int f = 0; f < 6; f++) {
+        REQUIRE(par->bc[f] >= 0 && par->bc[f] <= 3, "bc must be 0..3");
+        REQUIRE(par->taper[f] >= 0, "taper widths must be >= 0");
+    }
+    REQUIRE(par->taper[0] + par->taper[1] <= nz && par->taper[2] + par->taper[3] <= nx && par->taper[4] + par->taper[5] <= ny,
+            "opposite tapers overlap");
+    REQUIRE(par->snap_precision >= 0 && par->snap_precision <= 2, "snap_precision must be 0..2");
+    REQUIRE(physics == FWI3D_ACOUSTIC || par->free_surface_stiffness > 0 || true, "");
+    if (physics == FWI3D_ACOUSTIC) REQUIRE(par->free_surface_stiffness > 0, "free_surface_stiffness must be positive");
+}
+
+int fail(fwi3d_ctx* c, const Err& e) {
+    if (c) c->err = e.msg;
+    return e.code;
+}
+
+}  // namespace
+
+// ======================================================================================
+extern "C" {
+
+int fwi3d_create(fwi3d_ctx** out, int device) {
+    if (!out) return FWI3D_EINVAL;
+    *out = nullptr;
+    fwi3d_ctx* c = new fwi3d_ctx();
+    try {
+        int ndev = 0;
+        HIP_OK(hipGetDeviceCount(&ndev));
+        REQUIRE(device >= 0 && device < ndev, "device index out of range");
+        c->device = device;
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        load_tables(device);
+    } catch (const Err& e) {
+        int code = e.code;
+        delete c;
+        return code;
+    }
+    *out = c;
+    return FWI3D_OK;
+}
+
+int fwi3d_destroy(fwi3d_ctx* c) {
+    if (!c) return FWI3D_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+    c->fields.release(); c->model.release(); c->aux.release(); c->snaps.release();
+    c->tapers.release(); c->injtab.release(); c->exttab.release(); c->tmpio.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FWI3D_OK;
+}
+
+const char* fwi3d_last_error(const fwi3d_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fwi3d_injector_tables(int kind, const float* loc, int npts, int nx, int ny, int nz, float dx, float dy, float dz,
+                          float ox, float oy, float oz, int* xind, int* yind, int* zind, float* xw, float* yw,
+                          float* zw) {
+    if (!(kind == 0 || kind == 1) || npts < 0 || nx < 3 || ny < 3 || nz < 3) return FWI3D_EINVAL;
+    const int nw = kind ? 6 : 3;
+    // The arithmetic is the reference's single-precision one (injector.cpp:28-80):
+    // pivot and alpha in float, squares in double (std::pow(float,int)), H weights
+    // rounded to float. A position half-way between two nodes therefore picks the
+    // same pivot as the reference.
+    const float hc[4] = {(float)kHnorm[0], (float)kHnorm[1], (float)kHnorm[2], (float)kHnorm[3]};
+    auto hw = [&](int i, int n) -> float {
+        if (i < 4) return hc[i];
+        if (i >= n - 4) return hc[n - 1 - i];
+        return 1.0f;
+    };
+    auto one = [&](float x, float o, float d, int n, int* ind, float* w) -> bool {
+        const float r = std::round((x - o) / d);
+        if (!(r >= 0.f && r <= (float)(n - 1))) return false;
+        int piv = (int)r;
+        if (piv == 0) piv = 1;
+        if (piv == n - 1) piv = n - 2;
+        ind[0] = piv - 1; ind[1] = piv; ind[2] = piv + 1;
+        const float a = (x - d * piv - o) / d;
+        const double a2 = (double)a * (double)a;
+        w[0] = (float)((a2 - a) / (2 * d * hw(piv - 1, n)));
+        w[1] = (float)((-a2 + 1) / (d * hw(piv, n)));
+        w[2] = (float)((a2 + a) / (2 * d * hw(piv + 1, n)));
+        w[nw + 0] = (float)((a2 - a) / 2);
+        w[nw + 1] = (float)(-a2 + 1);
+        w[nw + 2] = (float)((a2 + a) / 2);
+        if (nw == 6) {
+            w[3] = (1 - 2 * a) / (2 * d * d * hw(piv - 1, n));
+            w[4] = 2 * a / (d * d * hw(piv, n));
+            w[5] = (-1 - 2 * a) / (2 * d * d * hw(piv + 1, n));
+            w[nw + 3] = (1 - 2 * a) / (2 * d);
+            w[nw + 4] = 2 * a / (d);
+            w[nw + 5] = (-1 - 2 * a) / (2 * d);
+        }
+        return true;
+    };
+    for (int i = 0; i < npts; i++) {
+        bool ok = one(loc[3 * i + 0], ox, dx, nx, xind + 3 * i, xw + 2 * nw * i) &&
+                  one(loc[3 * i + 1], oy, dy, ny, yind + 3 * i, yw + 2 * nw * i) &&
+                  one(loc[3 * i + 2], oz, dz, nz, zind + 3 * i, zw + 2 * nw * i);
+        if (!ok) return FWI3D_EINVAL;
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_propagate_device(fwi3d_ctx* c, int physics, int adj, const float* d_model, const float* d_src, float* d_rcv,
+                           const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
+                           int nx, int ny, int nz, float dx, float dy, float dz) {
+    if (!c) return FWI3D_EINVAL;
+    try {
+        validate(physics, par, nx, ny, nz, dx, dy, dz);
+        REQUIRE(d_model && d_src, "null model or source");
+        check_injector(inj, nx, ny, nz, "source");
+        if (!adj) {
+            check_injector(ext, nx, ny, nz, "receiver");
+            REQUIRE(d_rcv || ext->npts == 0, "null receiver buffer");
+        }
+        HIP_OK(hipSetDevice(c->device));
+        Geom g = make_geom(nx, ny, nz, dx, dy, dz);
+        if (physics == FWI3D_ACOUSTIC)
+            propagate_acoustic(c, adj, d_model, d_src, d_rcv, inj, ext, d_grad, par, g);
+        else
+            propagate_elastic(c, physics == FWI3D_VTI, adj, d_model, d_src, d_rcv, inj, ext, d_grad, par, g);
+        drain_events(c);
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int fwi3d_propagate(fwi3d_ctx* c, int physics, int adj, const float* model, const float* src, float* rcv,
+                    const fwi3d_injector* inj, const fwi3d_injector* ext, float* grad, const fwi3d_param* par, int nx,
+                    int ny, int nz, float dx, float dy, float dz) {
+    if (!c) return FWI3D_EINVAL;
+    try {
+        validate(physics, par, nx, ny, nz, dx, dy, dz);
+        REQUIRE(model && src && inj, "null model, source or injector");
+        HIP_OK(hipSetDevice(c->device));
+        const size_t N = (size_t)nx * ny * nz;
+        const int np = physics == FWI3D_ACOUSTIC ? 2 : (physics == FWI3D_ELASTIC ? 3 : 6);
+        const int ncomp = physics == FWI3D_ACOUSTIC ? 1 : 3;
+        const int ns = physics == FWI3D_ACOUSTIC ? 1 : (inj->kind ? 6 : 3);
+        const int ng = physics == FWI3D_ACOUSTIC ? 2 : 3;
+        const size_t nsrc = (size_t)ns * inj->npts * par->nt;
+        const size_t nrcv = adj ? 0 : (size_t)ncomp * (ext ? ext->npts : 0) * par->nt;
+        const size_t bytes = np * N * 4 + nsrc * 4 + nrcv * 4 + (grad ? ng * N * 8 : 0) + 4 * 256;
+        c->tmpio.ensure(bytes);
+        char* base = c->tmpio.as<char>();
+        auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+        float* dm = (float*)base;
+        float* ds = (float*)(base + align(np * N * 4));
+        float* dr = (float*)((char*)ds + align(nsrc * 4));
+        double* dg = (double*)((char*)dr + align(nrcv * 4));
+        HIP_OK(hipMemcpyAsync(dm, model, np * N * 4, hipMemcpyHostToDevice, c->stream));
+        if (nsrc) HIP_OK(hipMemcpyAsync(ds, src, nsrc * 4, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> g64;
+        if (grad) {
+            g64.resize(ng * N);
+            for (size_t i = 0; i < g64.size(); i++) g64[i] = grad[i];
+            HIP_OK(hipMemcpyAsync(dg, g64.data(), ng * N * 8, hipMemcpyHostToDevice, c->stream));
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        int rc = fwi3d_propagate_device(c, physics, adj, dm, ds, nrcv ? dr : nullptr, inj, ext, grad ? dg : nullptr, par,
+                                        nx, ny, nz, dx, dy, dz);
+        if (rc != FWI3D_OK) return rc;
+        if (nrcv) HIP_OK(hipMemcpy(rcv, dr, nrcv * 4, hipMemcpyDeviceToHost));
+        if (grad) {
+            HIP_OK(hipMemcpy(g64.data(), dg, ng * N * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < g64.size(); i++) grad[i] = (float)g64[i];
+        }
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int fwi3d_profile(fwi3d_ctx* c, int enable) {
+    if (!c) return FWI3D_EINVAL;
+    try {
+        drain_events(c);
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+    c->prof = enable != 0;
+    if (enable) {
+        c->prof_ms[0] = c->prof_ms[1] = 0;
+        c->prof_n[0] = c->prof_n[1] = 0;
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_profile_read(const fwi3d_ctx* c, int kernel, double* ms, long long* n) {
+    if (!c || kernel < 0 || kernel > 1) return FWI3D_EINVAL;
+    if (ms) *ms = c->prof_ms[kernel];
+    if (n) *n = c->prof_n[kernel];
+    return FWI3D_OK;
+}
+
+}  // extern "C"

@@ -125,8 +125,8 @@ int propagate_t(int physics, int adj, const T* model, const T* src, T* rcv, cons
         orc::escale<T>(x, M, g, dx, dy, dz, dt, bc);                                                                   \
         return 0;                                                                                                      \
     }                                                                                                                  \
-    int orc_injector_##S(int kind, const double* loc, int npts, int nx, int ny, int nz, double dx, double dy,         \
-                         double dz, double ox, double oy, double oz, int* xi, int* yi, int* zi, T* xw, T* yw, T* zw) { \
+    int orc_injector_##S(int kind, const T* loc, int npts, int nx, int ny, int nz, T dx, T dy, T dz, T ox, T oy,      \
+                         T oz, int* xi, int* yi, int* zi, T* xw, T* yw, T* zw) {                                       \
         auto I = orc::make_injector<T>(kind, loc, npts, orc::Grid{nx, ny, nz}, dx, dy, dz, ox, oy, oz);                \
         std::copy(I.xi.begin(), I.xi.end(), xi);                                                                       \
         std::copy(I.yi.begin(), I.yi.end(), yi);                                                                       \

@@ -144,11 +144,12 @@ class Injector:
 
     def __init__(self, kind, loc, shape, d, o, s="f"):
         ny, nx, nz = shape
-        loc = np.ascontiguousarray(np.asarray(loc, dtype=np.float64).reshape(-1, 3))
+        T = _dt(s)
+        cT = C.c_float if s == "f" else C.c_double
+        loc = np.ascontiguousarray(np.asarray(loc, dtype=T).reshape(-1, 3))
         self.npts = loc.shape[0]
         self.kind = kind
         nw = 6 if kind else 3
-        T = _dt(s)
         self.xi = np.zeros((self.npts, 3), np.int32)
         self.yi = np.zeros_like(self.xi)
         self.zi = np.zeros_like(self.xi)
@@ -156,10 +157,17 @@ class Injector:
         self.yw = np.zeros_like(self.xw)
         self.zw = np.zeros_like(self.xw)
         getattr(lib(), f"orc_injector_{s}")(kind, _ptr(loc), self.npts, nx, ny, nz,
-                                            C.c_double(d[0]), C.c_double(d[1]), C.c_double(d[2]),
-                                            C.c_double(o[0]), C.c_double(o[1]), C.c_double(o[2]),
+                                            cT(d[0]), cT(d[1]), cT(d[2]), cT(o[0]), cT(o[1]), cT(o[2]),
                                             _ptr(self.xi), _ptr(self.yi), _ptr(self.zi),
                                             _ptr(self.xw), _ptr(self.yw), _ptr(self.zw))
+
+    def as_double(self):
+        """Same stencil (indices and weights of this table) with FP64 weights, for FP64
+        oracle runs on the identical discrete geometry."""
+        import copy
+        o = copy.copy(self)
+        o.xw, o.yw, o.zw = (w.astype(np.float64) for w in (self.xw, self.yw, self.zw))
+        return o
 
     def cstruct(self):
         return Inj(self.npts, self.kind, self.xi.ctypes.data, self.yi.ctypes.data, self.zi.ctypes.data,

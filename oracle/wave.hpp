@@ -257,34 +257,37 @@ struct Injector {
 };
 
 template <class T>
-Injector<T> make_injector(int kind, const double* loc /*[npts][3] x,y,z*/, int npts, const Grid& g, double dx, double dy,
-                          double dz, double ox, double oy, double oz) {
+Injector<T> make_injector(int kind, const T* loc /*[npts][3] x,y,z*/, int npts, const Grid& g, T dx, T dy, T dz, T ox,
+                          T oy, T oz) {
+    // Arithmetic of findIndicesWeights (injector.cpp:28-80, 170-237) in data_t = T:
+    // pivot and alpha in T, squares through std::pow(T,int) (double), H weights in T.
     Injector<T> I;
     I.npts = npts;
     I.kind = kind;
     const int nw = I.nw();
     I.xi.resize(3 * npts); I.yi.resize(3 * npts); I.zi.resize(3 * npts);
     I.xw.assign(2 * nw * npts, 0); I.yw.assign(2 * nw * npts, 0); I.zw.assign(2 * nw * npts, 0);
-    auto axis_setup = [&](double x, double o, double d, int n, int* ind, T* w) {
-        int piv = (int)std::lround((x - o) / d);
+    auto H = [](int i, int n) -> T { return (T)hweight(i, n); };
+    auto axis_setup = [&](T x, T o, T d, int n, int* ind, T* w) {
+        int piv = (int)std::round((x - o) / d);
         if (piv == 0) piv = 1;
         if (piv == n - 1) piv = n - 2;
         ind[0] = piv - 1; ind[1] = piv; ind[2] = piv + 1;
-        double a = (x - d * piv - o) / d;
-        double H[3] = {hweight(piv - 1, n), hweight(piv, n), hweight(piv + 1, n)};
-        w[0] = (T)((a * a - a) / (2 * d * H[0]));
-        w[1] = (T)((1 - a * a) / (d * H[1]));
-        w[2] = (T)((a * a + a) / (2 * d * H[2]));
-        w[nw + 0] = (T)((a * a - a) / 2);
-        w[nw + 1] = (T)(1 - a * a);
-        w[nw + 2] = (T)((a * a + a) / 2);
+        T a = (x - d * piv - o) / d;
+        double a2 = std::pow(a, 2);
+        w[0] = (T)((a2 - a) / (2 * d * H(piv - 1, n)));
+        w[1] = (T)((-a2 + 1) / (d * H(piv, n)));
+        w[2] = (T)((a2 + a) / (2 * d * H(piv + 1, n)));
+        w[nw + 0] = (T)((a2 - a) / 2);
+        w[nw + 1] = (T)(-a2 + 1);
+        w[nw + 2] = (T)((a2 + a) / 2);
         if (nw == 6) {
-            w[3] = (T)((1 - 2 * a) / (2 * d * d * H[0]));
-            w[4] = (T)(2 * a / (d * d * H[1]));
-            w[5] = (T)((-1 - 2 * a) / (2 * d * d * H[2]));
-            w[nw + 3] = (T)((1 - 2 * a) / (2 * d));
-            w[nw + 4] = (T)(2 * a / d);
-            w[nw + 5] = (T)((-1 - 2 * a) / (2 * d));
+            w[3] = (1 - 2 * a) / (2 * d * d * H(piv - 1, n));
+            w[4] = 2 * a / (d * d * H(piv, n));
+            w[5] = (-1 - 2 * a) / (2 * d * d * H(piv + 1, n));
+            w[nw + 3] = (1 - 2 * a) / (2 * d);
+            w[nw + 4] = 2 * a / (d);
+            w[nw + 5] = (-1 - 2 * a) / (2 * d);
         }
     };
     for (int i = 0; i < npts; i++) {
@@ -375,7 +378,7 @@ void propagate_acoustic(bool adj, const T* model, const T* src, T* rcv, const In
         const T sdt = (T)par.sub * dt;
         const T* fn = snaps + (size_t)itl * N;
         const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * N : nullptr;
-        const T* fp = snaps + (size_t)(itl + 1) * N;
+        const T* fp = itl + 1 < n_snap_slots(nt, par.sub) ? snaps + (size_t)(itl + 1) * N : nullptr;
         T* t = tmp.data();
         d1<T>(AX_X, false, fn, t + 0 * N, g, dx);
         d1<T>(AX_Y, false, fn, t + 1 * N, g, dy);
@@ -391,11 +394,12 @@ void propagate_acoustic(bool adj, const T* model, const T* src, T* rcv, const In
             size_t i = (size_t)ii;
             T dot = t[3 * N + i] * t[i] + t[4 * N + i] * t[N + i] + t[5 * N + i] * t[2 * N + i];
             T k2 = sdt * K[i] * K[i];
+            const T fpv = fp ? fp[i] : (T)0;
             if (itl > 0 && itl < last) {
-                gK[i] -= (T)1.0 / k2 * adjf[i] * (fp[i] - 2 * fn[i] + fm[i]);
+                gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - 2 * fn[i] + fm[i]);
                 gB[i] += sdt * dot;
             } else if (itl == 0) {
-                gK[i] -= (T)1.0 / k2 * adjf[i] * (fp[i] - fn[i]);
+                gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - fn[i]);
                 gB[i] += (T)0.5 * sdt * dot;
             } else {
                 gK[i] -= (T)1.0 / k2 * adjf[i] * (-2 * fn[i] + fm[i]);
@@ -464,7 +468,7 @@ void propagate_elastic(bool vti, bool adj, const T* model, const T* src, T* rcv,
         const T sdt = (T)par.sub * dt;
         const T* fn = snaps + (size_t)itl * 3 * N;
         const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * 3 * N : nullptr;
-        const T* fp = snaps + (size_t)(itl + 1) * 3 * N;
+        const T* fp = itl + 1 < n_snap_slots(nt, par.sub) ? snaps + (size_t)(itl + 1) * 3 * N : nullptr;
         T* t = tmp.data();
         // forward-field derivatives: ux_x, uy_y, uz_z, uz_x, ux_z, uy_x, ux_y, uz_y, uy_z
         d1<T>(AX_X, false, fn + 0 * N, t + 0 * N, g, dx);
@@ -492,7 +496,8 @@ void propagate_elastic(bool vti, bool adj, const T* model, const T* src, T* rcv,
             T rt = 0;
             for (int c = 0; c < 3; c++) {
                 size_t q = c * N + i;
-                T d2t = branch == 1 ? fp[q] - 2 * fn[q] + fm[q] : (branch == 0 ? fp[q] - fn[q] : -fn[q] + fm[q]);
+                const T fpv = fp ? fp[q] : (T)0;
+                T d2t = branch == 1 ? fpv - 2 * fn[q] + fm[q] : (branch == 0 ? fpv - fn[q] : -fn[q] + fm[q]);
                 rt += adjf[q] * d2t;
             }
             if (!vti) {

@@ -1,0 +1,80 @@
+"""The C-ABI library loads and exports every function include/fwi3d_hip.h declares;
+host-side entry points that need no GPU behave like the reference."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fwi3d_amd as F
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "fwi3d_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fwi3d_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 8
+    L = F.lib()
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/fwi3d_hip.h but not exported"
+    assert set(F.EXPORTS) == set(names)
+
+
+def test_engine_fails_loudly_without_device():
+    # No GPU in the CPU suite: creating a context must raise, never fall back.
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError):
+        F.Engine(0)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_injector_tables_match_oracle(kind):
+    shape = (40, 37, 33)  # ny, nx, nz
+    d = (0.01, 0.012, 0.011)
+    o = (0.1, -0.05, 0.0)
+    rng = np.random.default_rng(7)
+    loc = np.stack([rng.uniform(o[0], o[0] + 36 * d[0], 50), rng.uniform(o[1], o[1] + 39 * d[1], 50),
+                    rng.uniform(o[2], o[2] + 32 * d[2], 50)], axis=1).astype(np.float32)
+    # include the exact edges and half-cell ties
+    loc[0] = (o[0], o[1], o[2])
+    loc[1] = (o[0] + 36 * d[0], o[1] + 39 * d[1], o[2] + 32 * d[2])
+    loc[2] = (o[0] + 10.5 * d[0], o[1] + 20.5 * d[1], o[2] + 5.5 * d[2])
+    a = F.Injector(kind, loc, shape, d, o)
+    b = O.Injector(kind, loc, shape, d, o, s="f")
+    assert np.array_equal(a.xind, b.xi) and np.array_equal(a.yind, b.yi) and np.array_equal(a.zind, b.zi)
+    for wa, wb in ((a.xw, b.xw), (a.yw, b.yw), (a.zw, b.zw)):
+        np.testing.assert_allclose(wa, wb, rtol=2e-6, atol=1e-6 * np.abs(wb).max())
+
+
+def test_injector_rejects_points_outside():
+    with pytest.raises(ValueError):
+        F.Injector(0, [[5.0, 0.1, 0.1]], (20, 20, 20), (0.01, 0.01, 0.01))
+
+
+def test_injector_moment_conditions():
+    # Injection weights reproduce the H-weighted moments of a delta: sum h H_i w_i (x_i-x)^k = delta_k0
+    n, h = 30, 0.01
+    for x in (0.0, 0.004, 0.013, 0.1234, 0.29):
+        inj = F.Injector(0, [[x, 0.1, 0.1]], (n, n, n), (h, h, h))
+        idx, w = inj.xind[0], inj.xw[0, 0].astype(np.float64)
+        H = np.array([O.hweight(i, n) for i in idx]) if hasattr(O, "hweight") else None
+        Hw = np.array([[17 / 48, 59 / 48, 43 / 48, 49 / 48][i] if i < 4 else
+                       ([17 / 48, 59 / 48, 43 / 48, 49 / 48][n - 1 - i] if i >= n - 4 else 1.0) for i in idx])
+        xi = idx * h
+        for k in range(3):
+            m = np.sum(h * Hw * w * (xi - x) ** k)
+            assert abs(m - (1.0 if k == 0 else 0.0)) < 1e-5 * (h ** k if k else 1) + 1e-9, (x, k, m)
+        # extraction weights reproduce quadratics exactly
+        we = inj.xw[0, 1].astype(np.float64)
+        for k in range(3):
+            assert abs(np.sum(we * xi ** k) - x ** k) < 1e-6

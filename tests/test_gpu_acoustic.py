@@ -1,0 +1,127 @@
+"""GPU parity of the acoustic path (nl_we_op_a::propagate, src/cpp/we_op.cpp:1607-1827)
+through the C ABI, against the oracle on the same inputs.
+
+Tolerances (north_star): traces relative L2 <= 1e-4 against the single-precision
+oracle; gradients relative L2 <= 1e-3."""
+import os
+
+import numpy as np
+import pytest
+
+import fwi3d_amd as F
+from fwi3d_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TRACE_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+def rel(a, b):
+    return float(np.linalg.norm((np.asarray(a, np.float64) - b).ravel()) / np.linalg.norm(np.asarray(b, np.float64).ravel()))
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = F.Engine(0)
+    yield e
+    e.close()
+
+
+def small_shot(shape=(36, 40, 44), nt=300, dt=0.001, h=0.01, seed=3):
+    """Smooth random acoustic model, a source off-centre and receivers spread over the
+    box including points next to every face."""
+    ny, nx, nz = shape
+    rng = np.random.default_rng(seed)
+    vp = (1.6 + 0.8 * W.smooth_random(shape, 4, seed)).astype(np.float32)
+    rho = W.gardner(vp) * (1 + 0.1 * W.smooth_random(shape, 3, seed + 1)).astype(np.float32)
+    model = W.acoustic_convert(vp, rho)
+    src = W.ricker(25.0, nt, dt)[None, None, :]
+    sloc = np.array([[0.47 * (nx - 1) * h, 0.52 * (ny - 1) * h, 0.37 * (nz - 1) * h]], np.float32)
+    L = np.array([(nx - 1) * h, (ny - 1) * h, (nz - 1) * h])
+    pts = [rng.uniform(0, 1, 3) * L for _ in range(40)]
+    for ax in range(3):
+        for end in (0.0, 1.0):
+            p = rng.uniform(0.2, 0.8, 3) * L
+            p[ax] = end * L[ax]
+            pts.append(p)
+    rloc = np.array(pts, np.float32)
+    return model, src, sloc, rloc, (h, h, h)
+
+
+CASES = {
+    "free+absorb+taper": dict(bc=(1, 2, 2, 2, 2, 2), taper=(0, 8, 8, 8, 8, 8)),
+    "absorb-all+taper": dict(bc=(2, 2, 2, 2, 2, 2), taper=(5, 5, 5, 5, 5, 5)),
+    "free-all": dict(bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6),
+    "mixed": dict(bc=(3, 2, 1, 3, 0, 2), taper=(0, 4, 0, 6, 3, 0)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_acoustic_forward_matches_oracle(eng, case):
+    model, src, sloc, rloc, d = small_shot()
+    cfg = CASES[case]
+    nt, dt = src.shape[-1], 0.001
+    shape = model.shape[-3:]
+    print(f"[acoustic fwd] case={case} shape={shape} nt={nt}", flush=True)
+    inj, ext = F.Injector(0, sloc, shape, d), F.Injector(0, rloc, shape, d)
+    par = F.make_param(nt, dt, bc=cfg["bc"], taper=cfg["taper"])
+    got = eng.propagate("acoustic", model, src, inj, ext, par, d)
+
+    oinj, oext = O.Injector(0, sloc, shape, d, (0, 0, 0)), O.Injector(0, rloc, shape, d, (0, 0, 0))
+    opar = O.make_par(nt, 0, dt, bc=cfg["bc"], taper=cfg["taper"])
+    want, _ = O.propagate("acoustic", model, src, oinj, oext, opar, d, s="f")
+    want64, _ = O.propagate("acoustic", model.astype(np.float64), src.astype(np.float64), oinj.as_double(),
+                            oext.as_double(), opar, d, s="d")
+    e32, e64, o64 = rel(got, want), rel(got, want64), rel(want, want64)
+    print(f"  rel L2 vs oracle f32 {e32:.2e}, vs f64 {e64:.2e} (oracle f32 vs f64 {o64:.2e})")
+    assert np.isfinite(got).all() and np.abs(want).max() > 0
+    assert e32 <= TRACE_TOL
+    assert e64 <= max(TRACE_TOL, 3 * o64)
+
+
+def test_acoustic_config_a_full_size(eng):
+    """Config A at full size (100^3, nt=1000) against the committed oracle traces."""
+    s = W.config_a()
+    print(f"[acoustic config A] shape={s.shape} nt={s.nt}", flush=True)
+    inj, ext = F.Injector(0, s.src_loc, s.shape, s.d), F.Injector(0, s.rcv_loc, s.shape, s.d)
+    par = F.make_param(s.nt, s.dt, bc=s.bc, taper=s.taper, taper_strength=s.taper_strength)
+    got = eng.propagate("acoustic", s.model, s.src, inj, ext, par, s.d)
+    want = np.load(os.path.join(GOLDEN, "config_a_traces_f32.npy"))
+    want64 = np.load(os.path.join(GOLDEN, "config_a_traces_f64.npy"))
+    e32, e64, o64 = rel(got, want), rel(got, want64), rel(want, want64)
+    print(f"  rel L2 vs oracle f32 {e32:.2e}, vs f64 {e64:.2e} (oracle f32 vs f64 {o64:.2e})")
+    assert e32 <= TRACE_TOL
+    assert e64 <= max(TRACE_TOL, 3 * o64)
+
+
+@pytest.mark.parametrize("sub", [3, 4])
+def test_acoustic_gradient_matches_oracle(eng, sub):
+    """Forward with snapshots every `sub` steps, then the adjoint with a residual
+    injected at the receivers; imaging condition (gK, g(1/rho)) vs the oracle.
+    nt=201 with sub=4 hits the (nt-1)%sub==0 'last sample' branch, sub=3 the
+    zero-slot branch."""
+    model, src, sloc, rloc, d = small_shot(nt=201)
+    nt, dt = 201, 0.001
+    shape = model.shape[-3:]
+    cfg = CASES["free+absorb+taper"]
+    print(f"[acoustic grad] sub={sub} shape={shape} nt={nt}", flush=True)
+    inj, ext = F.Injector(0, sloc, shape, d), F.Injector(0, rloc, shape, d)
+    par = F.make_param(nt, dt, sub=sub, bc=cfg["bc"], taper=cfg["taper"])
+    rcv = eng.propagate("acoustic", model, src, inj, ext, par, d)
+    res = np.ascontiguousarray(rcv[:, :, ::-1] * 0.7)  # any residual-shaped input
+    grad = np.zeros((2,) + shape, np.float32)
+    eng.propagate("acoustic", model, res, ext, None, par, d, adj=True, grad=grad)
+
+    oinj, oext = O.Injector(0, sloc, shape, d, (0, 0, 0)), O.Injector(0, rloc, shape, d, (0, 0, 0))
+    opar = O.make_par(nt, sub, dt, bc=cfg["bc"], taper=cfg["taper"])
+    orcv, snaps = O.propagate("acoustic", model, src, oinj, oext, opar, d, s="f")
+    assert rel(rcv, orcv) <= TRACE_TOL
+    ograd = np.zeros((2,) + shape, np.float32)
+    O.propagate("acoustic", model, res, oext, oinj, opar, d, adj=True, grad=ograd, snaps=snaps, s="f")
+    for k in range(2):
+        e = rel(grad[k], ograd[k])
+        print(f"  grad[{k}] rel L2 {e:.2e}")
+        assert e <= GRAD_TOL
