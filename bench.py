@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X wave-propagation / adjoint-gradient engine.
+
+Default workload (the configuration the headline metric is quoted on): elastic
+isotropic forward + adjoint gradient, 1 shot, 200^3 cells, h = 10 m, nt = 2000,
+dt = 0.8 ms, FP32 snapshots every 4 steps, 5-layer model with a +5% Gaussian vp
+anomaly, explosive 12 Hz Ricker at z = 30 m, 40x40 3-C receivers at z = 20 m,
+residual against data modelled without the anomaly.
+
+One step = one shot gradient: forward propagation with snapshots, residual, adjoint
+propagation with the imaging condition (all inputs resident in HBM before timing).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 200] [--nt 2000]
+
+Prints one JSON line (see the keys at the end of main()).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s & HBM GB/s vs roofline; shot-gradients/s, p50/p99 latency"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+ELASTIC_BYTES_PER_CELL = 48.0  # algorithmic bytes per elastic cell-update (u, u_prev, u_next x3 + lambda, mu, rho)
+
+
+def cpu_baseline(shot, seconds=15.0):
+    """Time the oracle (FP32 CPU restatement, OpenMP) on a bounded sample of the same
+    workload: forward time steps of the same model and geometry."""
+    from oracle import oracle as O
+    shape = shot.shape
+    inj = O.Injector(shot.src_kind, shot.src_loc, shape, shot.d, (0, 0, 0))
+    ext = O.Injector(0, shot.rcv_loc, shape, shot.d, (0, 0, 0))
+
+    def run(nsteps):
+        par = O.make_par(nsteps + 1, 0, shot.dt, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
+        src = np.ascontiguousarray(shot.src[..., :nsteps + 1])
+        t = time.perf_counter()
+        O.propagate(shot.physics, shot.model, src, inj, ext, par, shot.d)
+        return time.perf_counter() - t
+
+    t2 = run(2)
+    nsteps = int(max(2, min(200, seconds / max(t2 / 2, 1e-3))))
+    t = run(nsteps)
+    cells = float(np.prod(shape))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{nsteps} forward time steps of the same {shape[2]}x{shape[1]}x{shape[0]} {shot.physics} shot "
+                      f"(oracle FP32 C++/OpenMP, {t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=200)
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (shots would shard one per GPU); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fwi3d_amd as F
+    from fwi3d_amd import workloads as W
+
+    shot = W.config_b(n=args.n, nt=args.nt)
+    obs = W.config_b(n=args.n, nt=args.nt, anomaly=False)
+    shape = shot.shape
+    ny, nx, nz = shape
+    N = nx * ny * nz
+    nt = shot.nt
+    dev = torch.device("cuda:0")
+    eng = F.Engine(0)
+    inj = F.Injector(shot.src_kind, shot.src_loc, shape, shot.d)
+    ext = F.Injector(0, shot.rcv_loc, shape, shot.d)
+    par = F.make_param(nt, shot.dt, sub=shot.sub, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
+    par0 = F.make_param(nt, shot.dt, sub=0, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
+
+    model = torch.from_numpy(shot.model).to(dev)
+    model_obs = torch.from_numpy(obs.model).to(dev)
+    src = torch.from_numpy(np.ascontiguousarray(shot.src)).to(dev)
+    rcv = torch.zeros((3, ext.npts, nt), dtype=torch.float32, device=dev)
+    dobs = torch.zeros_like(rcv)
+    res = torch.zeros_like(rcv)
+    grad = torch.zeros((3, ny, nx, nz), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    # observed data (not timed)
+    eng.propagate_device("elastic", model_obs.data_ptr(), src.data_ptr(), dobs.data_ptr(), inj, ext, par0, shape, shot.d)
+
+    misfits = []
+
+    def shot_gradient():
+        eng.propagate_device("elastic", model.data_ptr(), src.data_ptr(), rcv.data_ptr(), inj, ext, par, shape, shot.d)
+        torch.cuda.synchronize()
+        r = rcv - dobs
+        # trapezoid misfit 0.5*dt*(sum r^2 - (r_0^2 + r_{nt-1}^2)/2) in FP64 (optimization.hpp:410-424)
+        r64 = r.double()
+        misfits.append(float(0.5 * shot.dt * ((r64 * r64).sum() - 0.5 * (r64[..., 0] ** 2 + r64[..., -1] ** 2).sum())))
+        # adjoint source: -alpha * time-reversed residual (Ht then Ht^-1 cancel; we_op.cpp:1001-1003)
+        res.copy_(-torch.flip(r, dims=[2]))
+        grad.zero_()
+        torch.cuda.synchronize()
+        eng.propagate_device("elastic", model.data_ptr(), res.data_ptr(), None, ext, None, par, shape, shot.d,
+                             adj=True, grad64_ptr=grad.data_ptr())
+
+    for _ in range(args.warmup):
+        shot_gradient()
+    torch.cuda.synchronize()
+    eng.profile(True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        shot_gradient()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step_ms, step_n = eng.profile_read(0)
+    flux_ms, flux_n = eng.profile_read(1)
+    eng.profile(False)
+
+    g = grad.cpu().numpy()
+    finite = bool(np.isfinite(g).all() and np.abs(g).max() > 0)
+
+    updates = 2.0 * N * (nt - 1) * args.steps  # forward + adjoint cell-updates
+    gcell = updates / elapsed / 1e9
+    t_step = step_ms / max(step_n, 1) / 1e3
+    t_flux = flux_ms / max(flux_n, 1) / 1e3
+    t_pair = t_step + t_flux
+    achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": round(gcell, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"elastic isotropic forward + adjoint gradient, 1 shot, {nz}x{nx}x{ny}, nt={nt}, "
+                               f"dt={shot.dt}, sub={shot.sub}, FP32 snapshots, FP64 gradient accumulation",
+                   "nz": nz, "nx": nx, "ny": ny, "nt": nt, "receivers": int(ext.npts) * 3},
+        "shot_gradients_per_s": round(args.steps / elapsed, 4),
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "misfit": misfits[-1] if misfits else None,
+        "gradient_finite": finite,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "elastic time step = elastic_flux + elastic_step launches",
+            "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
+            "avg_us": {"elastic_step": round(t_step * 1e6, 2), "elastic_flux": round(t_flux * 1e6, 2)},
+            "launches": {"elastic_step": step_n, "elastic_flux": flux_n},
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(shot)
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, 'item') else str(o)), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -178,7 +178,8 @@ class Engine:
         self._check(lib().fwi3d_profile(self._h, int(bool(enable))), "fwi3d_profile")
 
     def profile_read(self, kernel=0):
-        """(milliseconds, launches) of kernel class 0 (time step) or 1 (flux pre-pass)."""
+        """(milliseconds, launches) of kernel class 0 (interior time step), 1 (flux
+        pre-pass, elastic/VTI) or 2 (boundary-shell time step)."""
         ms = C.c_double()
         n = C.c_longlong()
         self._check(lib().fwi3d_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)), "fwi3d_profile_read")

@@ -12,12 +12,14 @@ namespace fwi3d {
 
 // Wavefield / model layout in HBM: [y][x][z] with the z row padded to a pitch `pz`
 // that is a multiple of 64 floats (one 256-byte wavefront load per row segment).
+// Offsets inside one field (and inside [3][field] arrays) are 32-bit: the host
+// rejects grids with 3 * n >= 2^31.
 struct Geom {
     int nx, ny, nz, pz;
-    long long sx, sy;  // element strides of x and y (z stride is 1)
-    long long n;       // padded elements per field: ny * sy
+    int sx, sy;  // element strides of x and y (z stride is 1)
+    int n;       // padded elements per field: ny * sy
     float dx, dy, dz;
-    __host__ __device__ long long at(int ix, int iy, int iz) const { return (long long)iy * sy + (long long)ix * sx + iz; }
+    __host__ __device__ int at(int ix, int iy, int iz) const { return iy * sy + ix * sx + iz; }
 };
 
 // D2 closure C[row][u-point][b-point], D1 closure rows, S and H weights.
@@ -51,19 +53,15 @@ __device__ __forceinline__ float d2_interior(float um2, float um1, float u0, flo
            (D2C7 * bm2 + D2C8 * bm1 + D2C9 * b0) * um2 + (D2C7 * bp2 + D2C8 * bp1 + D2C9 * b0) * up2;
 }
 
-// D2(b) (times h^2) at axis index i (any row) reading global memory: the field and
-// coefficient at point p, axis length n, element stride s.
-__device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, long long p, int i,
-                                          int n, long long s) {
-    if (i >= 6 && i < n - 6) {
-        return d2_interior(u[p - 2 * s], u[p - s], u[p], u[p + s], u[p + 2 * s], b[p - 2 * s], b[p - s], b[p],
-                           b[p + s], b[p + 2 * s]);
-    }
+// D2(b) (times h^2) on a boundary row (i < 6 or i >= n-6), reading global memory.
+// Used by the boundary-shell kernels only.
+__device__ __forceinline__ float d2_closure(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
+                                            int s) {
     const bool lo = i < 6;
     const int r = lo ? i : n - 1 - i;
-    // position of closure point k: lo -> k, hi -> n-1-k (relative to p: k-i or n-1-k-i)
-    const long long base = lo ? p - (long long)i * s : p + (long long)(n - 1 - i) * s;
-    const long long step = lo ? s : -s;
+    // position of closure point k: lo -> k, hi -> n-1-k
+    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
+    const int step = lo ? s : -s;
     float bk[8], uk[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -81,36 +79,37 @@ __device__ __forceinline__ float d2_point(const float* __restrict__ u, const flo
     return acc;
 }
 
-// D1 (times h) at axis index i reading global memory; optional coefficient `m`
-// multiplies the field inside the derivative.
-__device__ __forceinline__ float d1_point(const float* __restrict__ u, long long p, int i, int n, long long s) {
-    if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);
+// D2(b) (times h^2) at axis index i (any row) reading global memory: the field and
+// coefficient at point p, axis length n, element stride s.
+__device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
+                                          int s) {
+    if (i >= 6 && i < n - 6) {
+        return d2_interior(u[p - 2 * s], u[p - s], u[p], u[p + s], u[p + 2 * s], b[p - 2 * s], b[p - s], b[p],
+                           b[p + s], b[p + 2 * s]);
+    }
+    return d2_closure(u, b, p, i, n, s);
+}
+
+// D1 (times h) on a boundary row (i < 4 or i >= n-4), reading global memory.
+__device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, int i, int n, int s) {
     const bool lo = i < 4;
     const int r = lo ? i : n - 1 - i;
-    const long long base = lo ? p - (long long)i * s : p + (long long)(n - 1 - i) * s;
-    const long long step = lo ? s : -s;
+    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
+    const int step = lo ? s : -s;
     float acc = 0.f;
 #pragma unroll
     for (int j = 0; j < 6; j++) acc += c_d1b[r][j] * u[base + j * step];
     return lo ? acc : -acc;
 }
 
-__device__ __forceinline__ float d1_point_m(const float* __restrict__ u, const float* __restrict__ m, long long p, int i,
-                                            int n, long long s) {
-    if (i >= 4 && i < n - 4)
-        return D1C1 * (u[p + s] * m[p + s] - u[p - s] * m[p - s]) + D1C2 * (u[p + 2 * s] * m[p + 2 * s] - u[p - 2 * s] * m[p - 2 * s]);
-    const bool lo = i < 4;
-    const int r = lo ? i : n - 1 - i;
-    const long long base = lo ? p - (long long)i * s : p + (long long)(n - 1 - i) * s;
-    const long long step = lo ? s : -s;
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 6; j++) acc += c_d1b[r][j] * (u[base + j * step] * m[base + j * step]);
-    return lo ? acc : -acc;
+// D1 (times h) at axis index i reading global memory.
+__device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, int i, int n, int s) {
+    if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);
+    return d1_closure(u, p, i, n, s);
 }
 
 // Outward boundary derivative S (times h) at a face point p, `inward` = +/- stride.
-__device__ __forceinline__ float sat_s(const float* __restrict__ u, long long p, long long inward) {
+__device__ __forceinline__ float sat_s(const float* __restrict__ u, int p, int inward) {
     return c_sat_s[0] * u[p] + c_sat_s[1] * u[p + inward] + c_sat_s[2] * u[p + 2 * inward] + c_sat_s[3] * u[p + 3 * inward];
 }
 
@@ -123,8 +122,56 @@ struct Bound {
 };
 
 // taper: ((v * fz) * fx) * fy, the order of the reference's six taper calls.
-__device__ __forceinline__ float taper_apply(float v, const Bound& B, int ix, int iy, int iz) {
-    return ((v * B.tz2[iz]) * B.tx2[ix]) * B.ty2[iy];
+__device__ __forceinline__ float taper_apply(float v, const float* __restrict__ tz2, const float* __restrict__ tx2,
+                                             const float* __restrict__ ty2, int ix, int iy, int iz) {
+    return ((v * tz2[iz]) * tx2[ix]) * ty2[iy];
+}
+
+// Workgroup -> tile remap so that the workgroups one XCD receives (hardware assigns
+// workgroup i to XCD i % 8) are spatially contiguous.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = bid & 7, k = bid >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Boundary shell = points within `w` cells of any face. Enumerated as: the 2w
+// y-planes at each end (all x, z), then inside them the 2w x-rows (all z), then the
+// 2w z-points of the remaining columns. Returns false past the end.
+__device__ __forceinline__ bool shell_point(int t, int w, int nx, int ny, int nz, int& ix, int& iy, int& iz) {
+    const int a = 2 * w * nx * nz;
+    if (t < a) {
+        const int pl = t / (nx * nz), r = t - pl * (nx * nz);
+        iy = pl < w ? pl : ny - 2 * w + pl;
+        ix = r / nz;
+        iz = r - ix * nz;
+        return true;
+    }
+    t -= a;
+    const int nyi = ny - 2 * w;
+    const int b = nyi * 2 * w * nz;
+    if (t < b) {
+        const int pl = t / (2 * w * nz), r = t - pl * (2 * w * nz);
+        iy = w + pl;
+        const int xr = r / nz;
+        ix = xr < w ? xr : nx - 2 * w + xr;
+        iz = r - xr * nz;
+        return true;
+    }
+    t -= b;
+    const int nxi = nx - 2 * w;
+    if (t < nyi * nxi * 2 * w) {
+        const int col = t / (2 * w), k = t - col * (2 * w);
+        iy = w + col / nxi;
+        ix = w + col % nxi;
+        iz = k < w ? k : nz - 2 * w + k;
+        return true;
+    }
+    return false;
+}
+
+__host__ __device__ inline int shell_count(int w, int nx, int ny, int nz) {
+    return 2 * w * nx * nz + (ny - 2 * w) * 2 * w * nz + (ny - 2 * w) * (nx - 2 * w) * 2 * w;
 }
 
 }  // namespace fwi3d

@@ -88,11 +88,13 @@ struct fwi3d_ctx {
     long long snap_n = 0;
     bool snap_valid = false;
     int snap_dims[3] = {0, 0, 0};
-    // profiling: [0] time-step kernel, [1] flux pre-pass (elastic/VTI)
+    // profiling: [0] interior time-step kernel, [1] flux pre-pass (elastic/VTI),
+    // [2] boundary-shell time-step kernel
     bool prof = false;
-    double prof_ms[2] = {0, 0};
-    long long prof_n[2] = {0, 0};
+    double prof_ms[3] = {0, 0, 0};
+    long long prof_n[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
     std::vector<int> ev_kind;
 };
 
@@ -102,9 +104,11 @@ Geom make_geom(int nx, int ny, int nz, float dx, float dy, float dz) {
     Geom g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.pz = (nz + 63) / 64 * 64;
+    const long long n = (long long)ny * nx * g.pz;
+    REQUIRE(9 * n < (1LL << 31), "grid too large for 32-bit field offsets (9 * padded cells must stay below 2^31)");
     g.sx = g.pz;
-    g.sy = (long long)nx * g.pz;
-    g.n = (long long)ny * g.sy;
+    g.sy = nx * g.pz;
+    g.n = (int)n;
     g.dx = dx; g.dy = dy; g.dz = dz;
     return g;
 }
@@ -131,10 +135,6 @@ void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g)
                                 g.nz * sizeof(float), g.nz * sizeof(float), (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,
                                 c->stream));
 }
-
-struct DevInj {
-    InjTab t;
-};
 
 // Upload injector tables into `buf`: ints then floats.
 InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
@@ -174,56 +174,48 @@ void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char*
     }
 }
 
-// Event-bracketed launch of a step kernel when profiling.
+// Event-bracketed launch of a step kernel when profiling. Events come from a pool
+// that is created once and reused, so the timed region records but never creates.
 template <class F>
 void timed(fwi3d_ctx* c, int kind, F&& launch) {
     if (!c->prof) {
         launch();
         return;
     }
-    hipEvent_t a, b;
-    HIP_OK(hipEventCreate(&a));
-    HIP_OK(hipEventCreate(&b));
-    HIP_OK(hipEventRecord(a, c->stream));
+    if (c->ev_used + 2 > c->ev.size()) {
+        for (int k = 0; k < 2; k++) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            c->ev.push_back(e);
+        }
+    }
+    HIP_OK(hipEventRecord(c->ev[c->ev_used], c->stream));
     launch();
-    HIP_OK(hipEventRecord(b, c->stream));
-    c->ev.push_back(a);
-    c->ev.push_back(b);
+    HIP_OK(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
     c->ev_kind.push_back(kind);
+    c->ev_used += 2;
 }
 
 void drain_events(fwi3d_ctx* c) {
-    if (c->ev.empty()) return;
+    if (c->ev_used == 0) return;
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i + 1 < c->ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
         float ms = 0;
         HIP_OK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
         const int k = c->ev_kind[i / 2];
         c->prof_ms[k] += ms;
         c->prof_n[k] += 1;
-        (void)hipEventDestroy(c->ev[i]);
-        (void)hipEventDestroy(c->ev[i + 1]);
     }
-    c->ev.clear();
+    c->ev_used = 0;
     c->ev_kind.clear();
 }
 
 // --------------------------------------------------------------------------------------
-void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const float* d_src, float* d_rcv,
-                        const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
-                        const Geom& g) {
-    const long long n = g.n;
-    c->fields.ensure(3 * n * sizeof(float));
-    c->model.ensure(2 * n * sizeof(float));
-    float* F = c->fields.as<float>();
-    float* M = c->model.as<float>();
-    HIP_OK(hipMemsetAsync(F, 0, 3 * n * sizeof(float), c->stream));
-    HIP_OK(hipMemsetAsync(M, 0, 2 * n * sizeof(float), c->stream));
-    to_padded(c, M, d_model, 2, g);
-    const float* K = M;
-    const float* B = M + n;
+struct Tapers {
+    const float *tz2, *tx2, *ty2;
+};
 
-    // tapers
+Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
     std::vector<float> tz, tx, ty;
     taper_table(tz, g.nz, par->taper[0], par->taper[1], par->taper_strength);
     taper_table(tx, g.nx, par->taper[2], par->taper[3], par->taper_strength);
@@ -235,9 +227,31 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, c->stream));
-    Bound Bd;
-    for (int f = 0; f < 6; f++) Bd.bc[f] = par->bc[f];
-    Bd.tz2 = dtz; Bd.tx2 = dtx; Bd.ty2 = dty;
+    return Tapers{dtz, dtx, dty};
+}
+
+// Tiling of the y-marching kernels: 64(z) x TX(x) column tiles, y in chunks.
+template <class Args>
+int tile_grid(Args& A, const Geom& g, int TX) {
+    A.ntz = (g.nz + 63) / 64;
+    A.ntx = (g.nx + TX - 1) / TX;
+    A.ychunk = std::max(4, std::min(g.ny, 16));
+    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
+    return A.ntz * A.ntx * A.nyc;
+}
+
+void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const float* d_src, float* d_rcv,
+                        const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
+                        const Geom& g) {
+    const long long n = g.n;
+    c->fields.ensure(3 * n * sizeof(float));
+    c->model.ensure(2 * n * sizeof(float));
+    float* F = c->fields.as<float>();
+    float* M = c->model.as<float>();
+    HIP_OK(hipMemsetAsync(F, 0, 3 * n * sizeof(float), c->stream));
+    HIP_OK(hipMemsetAsync(M, 0, 2 * n * sizeof(float), c->stream));
+    to_padded(c, M, d_model, 2, g);
+    const Tapers tp = setup_tapers(c, par, g);
 
     InjTab I = upload_inj(c, c->injtab, inj);
     InjTab E{};
@@ -260,26 +274,23 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
                 "adjoint propagation needs the snapshots of a matching forward propagation on this context");
     }
     float* snaps = c->snaps.as<float>();
-
     float* prev = F;
     float* curr = F + n;
     float* next = F + 2 * n;
 
-    const int nthr = 256;
-    if (I.npts > 0) {
-        int nb = (I.npts * 27 + nthr - 1) / nthr;
-        hipLaunchKernelGGL(acoustic_inject, dim3(nb), dim3(nthr), 0, c->stream, prev, K, B, g, Bd, I, d_src, nt, 0, dt, 1);
-    }
-
     constexpr int TX = 4;
-    AcuArgs A;
-    A.K = K; A.b = B; A.g = g; A.B = Bd; A.dt = dt; A.dt2 = dt * dt;
+    AcuArgs A{};
+    A.K = M; A.b = M + n; A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2; A.g = g;
+    for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
+    A.dt = dt; A.dt2 = dt * dt;
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
-    A.ntz = (g.nz + 63) / 64;
-    A.ntx = (g.nx + TX - 1) / TX;
-    A.ychunk = std::max(4, std::min(g.ny, 16));
-    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
-    const int nblk = A.ntz * A.ntx * A.nyc;
+    const int nblk = tile_grid(A, g, TX);
+    const int nshell = shell_count(6, g.nx, g.ny, g.nz);
+
+    const int nthr = 256;
+    const int nbi = (I.npts * 27 + nthr - 1) / nthr;
+    if (I.npts > 0)
+        hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, prev, A, I, d_src, nt, 0, 1);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     double* gK = d_grad;
@@ -293,7 +304,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         // a slot past the store reads as zero (the reference dereferences a null slot there)
         const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * n : nullptr;
         dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
-        hipLaunchKernelGGL(acoustic_grad, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, K, g, sdt, branch, gK, gB, 0LL);
+        hipLaunchKernelGGL(acoustic_grad, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, (const float*)M, g, sdt, branch, gK, gB);
     };
 
     for (int it = 0; it < nt - 1; it++) {
@@ -304,11 +315,10 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_step<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-        if (I.npts > 0) {
-            int nb = (I.npts * 27 + nthr - 1) / nthr;
-            hipLaunchKernelGGL(acoustic_inject, dim3(nb), dim3(nthr), 0, c->stream, next, K, B, g, Bd, I, d_src, nt, it, dt, 0);
-        }
+        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_interior<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_boundary, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+        if (I.npts > 0)
+            hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, next, A, I, d_src, nt, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
     }
     if (fwd_snap && (nt - 1) % sub == 0)
@@ -327,22 +337,6 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
     if (i < n) c11b[i] = 2.f * (1.f + 2.f * eps[i]) * mu[i];
 }
 
-void setup_bound(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g, Bound& Bd) {
-    std::vector<float> tz, tx, ty;
-    taper_table(tz, g.nz, par->taper[0], par->taper[1], par->taper_strength);
-    taper_table(tx, g.nx, par->taper[2], par->taper[3], par->taper_strength);
-    taper_table(ty, g.ny, par->taper[4], par->taper[5], par->taper_strength);
-    c->tapers.ensure((tz.size() + tx.size() + ty.size()) * sizeof(float));
-    float* dtz = c->tapers.as<float>();
-    float* dtx = dtz + tz.size();
-    float* dty = dtx + tx.size();
-    HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, c->stream));
-    for (int f = 0; f < 6; f++) Bd.bc[f] = par->bc[f];
-    Bd.tz2 = dtz; Bd.tx2 = dtx; Bd.ty2 = dty;
-}
-
 void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, const float* d_src, float* d_rcv,
                        const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
                        const Geom& g) {
@@ -358,21 +352,24 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     HIP_OK(hipMemsetAsync(F, 0, 9 * n * sizeof(float), c->stream));
     HIP_OK(hipMemsetAsync(Mf, 0, nmf * n * sizeof(float), c->stream));
     to_padded(c, Mf, d_model, npar, g);
-    EModel M{};
-    M.lam = Mf; M.mu = Mf + n; M.rho = Mf + 2 * n;
+    EArgs A{};
+    A.lam = Mf; A.mu = Mf + n; A.rho = Mf + 2 * n;
     if (vti) {
-        M.c13 = Mf + 3 * n; M.eps = Mf + 4 * n; M.c66 = Mf + 5 * n; M.c11b = Mf + 6 * n;
-        hipLaunchKernelGGL(vti_c11b_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, Mf + 6 * n, M.eps, M.mu, n);
+        A.c13 = Mf + 3 * n; A.eps = Mf + 4 * n; A.c66 = Mf + 5 * n; A.c11b = Mf + 6 * n;
+        hipLaunchKernelGGL(vti_c11b_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, Mf + 6 * n, A.eps, A.mu, n);
     }
-    Bound Bd;
-    setup_bound(c, par, g, Bd);
+    const Tapers tp = setup_tapers(c, par, g);
+    A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2;
+    A.g = g;
+    for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
+    A.aux = c->aux.as<float>();
     InjTab I = upload_inj(c, c->injtab, inj);
-    REQUIRE(adj || I.kind == 1 || true, "");
     InjTab E{};
     if (!adj) E = upload_inj(c, c->exttab, ext);
 
     const int nt = par->nt, sub = par->sub;
     const float dt = par->dt;
+    A.dt = dt; A.dt2 = dt * dt;
     const bool fwd_snap = !adj && sub > 0;
     const long long slot = 3 * n;
     if (fwd_snap) {
@@ -400,21 +397,16 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (I.npts == 0) return;
         for (int cc = 0; cc < 3; cc++) {
             if (vti)
-                hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, M, g, Bd, I, d_src, nt, it, dt, cc, mode);
+                hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, A, I, d_src, nt, it, cc, mode);
             else
-                hipLaunchKernelGGL(elastic_inject<false>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, M, g, Bd, I, d_src, nt, it, dt, cc, mode);
+                hipLaunchKernelGGL(elastic_inject<false>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, A, I, d_src, nt, it, cc, mode);
         }
     };
     inject(prev, 0, 1);
 
     constexpr int TX = 4;
-    EArgs A;
-    A.m = M; A.g = g; A.B = Bd; A.dt = dt; A.dt2 = dt * dt; A.aux = c->aux.as<float>();
-    A.ntz = (g.nz + 63) / 64;
-    A.ntx = (g.nx + TX - 1) / TX;
-    A.ychunk = std::max(4, std::min(g.ny, 16));
-    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
-    const int nblk = A.ntz * A.ntx * A.nyc;
+    const int nblk = tile_grid(A, g, TX);
+    const int nshell = shell_count(6, g.nx, g.ny, g.nz);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
@@ -426,9 +418,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * slot : nullptr;
         dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
         if (vti)
-            hipLaunchKernelGGL(elastic_grad<true>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, M, g, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+            hipLaunchKernelGGL(elastic_grad<true>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
         else
-            hipLaunchKernelGGL(elastic_grad<false>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, M, g, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+            hipLaunchKernelGGL(elastic_grad<false>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
     };
     auto extract = [&](const float* u, int it) {
         if (adj || E.npts == 0) return;
@@ -445,10 +437,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         A.prev_tapered = it > 0 ? 1 : 0;
         if (vti) {
             timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<true>, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
         } else {
             timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<false>, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
@@ -655,14 +649,16 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
     }
     c->prof = enable != 0;
     if (enable) {
-        c->prof_ms[0] = c->prof_ms[1] = 0;
-        c->prof_n[0] = c->prof_n[1] = 0;
+        for (int k = 0; k < 3; k++) {
+            c->prof_ms[k] = 0;
+            c->prof_n[k] = 0;
+        }
     }
     return FWI3D_OK;
 }
 
 int fwi3d_profile_read(const fwi3d_ctx* c, int kernel, double* ms, long long* n) {
-    if (!c || kernel < 0 || kernel > 1) return FWI3D_EINVAL;
+    if (!c || kernel < 0 || kernel > 2) return FWI3D_EINVAL;
     if (ms) *ms = c->prof_ms[kernel];
     if (n) *n = c->prof_n[kernel];
     return FWI3D_OK;

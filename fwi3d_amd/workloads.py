@@ -132,6 +132,7 @@ def config_b(n=200, nt=2000, dt=0.0008, h=0.01, anomaly=True, sub=4, nrec=40):
     Ricker at z=30 m, 40x40 three-component receivers at z=20 m (50 m spacing)."""
     vp, vs, rho = layered_elastic(n, h, anomaly)
     c = (n - 1) * h / 2
+    nrec = min(nrec, int(((n - 1) * h - 0.04) / 0.05) + 1)  # the receiver spread stays inside small grids
     w = ricker(12.0, nt, dt)
     src = np.zeros((6, 1, nt), F32)
     src[0, 0] = w  # mxx

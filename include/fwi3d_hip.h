@@ -98,8 +98,8 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
 /* Kernel timing with HIP events on the context's stream, for roofline accounting
  * (no counterpart in the reference). enable=1 starts collecting (and resets), 0 stops.
  * read() returns the summed milliseconds and launch count since the last enable of
- * kernel class `kernel`: 0 = the fused time-step kernel, 1 = the elastic/VTI flux
- * pre-pass. */
+ * kernel class `kernel`: 0 = the interior time-step kernel, 1 = the elastic/VTI flux
+ * pre-pass, 2 = the boundary-shell time-step kernel. */
 int fwi3d_profile(fwi3d_ctx *ctx, int enable);
 int fwi3d_profile_read(const fwi3d_ctx *ctx, int kernel, double *ms, long long *launches);
 
