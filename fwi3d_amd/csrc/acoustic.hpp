@@ -70,32 +70,31 @@ __global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
     const int col = cx * sx + cz;
     const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
     auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
-    // halo sources: x rows x0-2+lx / x0+TX+lx for lx < 2, z columns z0-2+lz / z0+TZ+lz for lz < 2
-    const int hx0 = max(0, x0 - 2 + lx) * sx + cz, hx1 = min(nx - 1, x0 + TX + lx) * sx + cz;
-    const int hz0 = cx * sx + max(0, z0 - 2 + lz), hz1 = cx * sx + min(nz - 1, z0 + TZ + lz);
+    HaloMap<TX> H;
+    H.init(lx, lz, x0, z0, nx, nz, sx);
 
-    float um2 = ldy(curr, y0 - 2), um1 = ldy(curr, y0 - 1), u0 = ldy(curr, y0), up1 = ldy(curr, y0 + 1), up2;
-    float bm2 = ldy(bb, y0 - 2), bm1 = ldy(bb, y0 - 1), b0 = ldy(bb, y0), bp1 = ldy(bb, y0 + 1), bp2;
+    // y queues y-2..y+2 plus the plane y+3, loaded one iteration ahead of its use
+    float um2 = ldy(curr, y0 - 2), um1 = ldy(curr, y0 - 1), u0 = ldy(curr, y0), up1 = ldy(curr, y0 + 1), up2 = ldy(curr, y0 + 2), up3;
+    float bm2 = ldy(bb, y0 - 2), bm1 = ldy(bb, y0 - 1), b0 = ldy(bb, y0), bp1 = ldy(bb, y0 + 1), bp2 = ldy(bb, y0 + 2), bp3;
+    // halo values of the plane to stage next, loaded one iteration ahead
+    float hxu = 0.f, hxb = 0.f, hzu = 0.f, hzb = 0.f;
+    auto prefetch = [&](int y) {
+        const int pl = min(y, ny - 1) * sy;
+        if (H.has_x) { hxu = curr[pl + H.xsrc]; hxb = bb[pl + H.xsrc]; }
+        if (H.has_z) { hzu = curr[pl + H.zsrc]; hzb = bb[pl + H.zsrc]; }
+    };
+    prefetch(y0);
 
     for (int iy = y0; iy < y1; iy++) {
-        up2 = ldy(curr, iy + 2);
-        bp2 = ldy(bb, iy + 2);
+        up3 = ldy(curr, iy + 3);
+        bp3 = ldy(bb, iy + 3);
         const int pl = iy * sy;
         __syncthreads();
         su[lx + 2][lz + 2] = u0;
         sb[lx + 2][lz + 2] = b0;
-        if (lx < 2) {
-            su[lx][lz + 2] = curr[pl + hx0];
-            sb[lx][lz + 2] = bb[pl + hx0];
-            su[TX + 2 + lx][lz + 2] = curr[pl + hx1];
-            sb[TX + 2 + lx][lz + 2] = bb[pl + hx1];
-        }
-        if (lz < 2) {
-            su[lx + 2][lz] = curr[pl + hz0];
-            sb[lx + 2][lz] = bb[pl + hz0];
-            su[lx + 2][TZ + 2 + lz] = curr[pl + hz1];
-            sb[lx + 2][TZ + 2 + lz] = bb[pl + hz1];
-        }
+        if (H.has_x) { su[H.xrow][H.xcol] = hxu; sb[H.xrow][H.xcol] = hxb; }
+        if (H.has_z) { su[H.zrow][H.zcol] = hzu; sb[H.zrow][H.zcol] = hzb; }
+        prefetch(iy + 1);
         __syncthreads();
         if (comp_xz && iy >= 6 && iy < ny - 6) {
             const int p = pl + col;
@@ -112,8 +111,8 @@ __global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
             const float nv = dt2 * lap * K[p] + 2.f * u0 - pv;
             next[p] = taper_apply(nv, tz2, tx2, ty2, ix, iy, iz);
         }
-        um2 = um1; um1 = u0; u0 = up1; up1 = up2;
-        bm2 = bm1; bm1 = b0; b0 = bp1; bp1 = bp2;
+        um2 = um1; um1 = u0; u0 = up1; up1 = up2; up2 = up3;
+        bm2 = bm1; bm1 = b0; b0 = bp1; bp1 = bp2; bp2 = bp3;
     }
 }
 
@@ -147,6 +146,7 @@ __global__ __launch_bounds__(256) void acoustic_boundary(AcuArgs A) {
     float pv = A.prev[p];
     if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
     float sat = 0.f;
+#pragma unroll
     for (int f = 0; f < 6; f++) {
         const int bc = A.bc[f];
         if (bc == 0) continue;
@@ -232,11 +232,13 @@ __global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, co
     atomicAdd(out + p, add);
 }
 
-// rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract).
-__global__ void extract_points(const float* __restrict__ u, Geom g, InjTab I, float* __restrict__ rcv, int nt, int it,
-                               int comp) {
+// rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract);
+// component comp = blockIdx.y reads field u + comp * g.n.
+__global__ void extract_points(const float* __restrict__ u3, Geom g, InjTab I, float* __restrict__ rcv, int nt, int it) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= I.npts) return;
+    const int comp = blockIdx.y;
+    const float* __restrict__ u = u3 + comp * g.n;
     const float* xw = I.xw + r * 2 * I.nw + I.nw;
     const float* yw = I.yw + r * 2 * I.nw + I.nw;
     const float* zw = I.zw + r * 2 * I.nw + I.nw;

@@ -135,6 +135,32 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Halo staging map of the y-marching kernels: a 64(z) x TX(x) tile staged in LDS
+// as [TX+4][64+4] with 2-cell halos. The 4 x-halo rows (4 x 64 values) and the
+// TX x 4 z-halo values are spread over the block's threads (thread h = lx*64+lz):
+// every thread loads at most one x-halo and one z-halo value per plane.
+template <int TX>
+struct HaloMap {
+    bool has_x, has_z;
+    int xrow, xcol, xsrc;  // LDS row/column and in-plane global offset of the x-halo value
+    int zrow, zcol, zsrc;  // same for the z-halo value
+    __device__ void init(int lx, int lz, int x0, int z0, int nx, int nz, int sx) {
+        const int h = lx * 64 + lz;
+        has_x = h < 256;
+        const int r4 = (h >> 6) & 3;
+        xrow = r4 < 2 ? r4 : TX + r4;
+        xcol = (h & 63) + 2;
+        const int gx = r4 < 2 ? x0 - 2 + r4 : x0 + TX + r4 - 2;
+        xsrc = max(0, min(nx - 1, gx)) * sx + min(z0 + (h & 63), nz - 1);
+        has_z = h < 4 * TX;
+        const int zr = (h >> 2) % TX, zc = h & 3;
+        zrow = zr + 2;
+        zcol = zc < 2 ? zc : 64 + zc;
+        const int gz = zc < 2 ? z0 - 2 + zc : z0 + 64 + zc - 2;
+        zsrc = min(x0 + zr, nx - 1) * sx + max(0, min(nz - 1, gz));
+    }
+};
+
 // Boundary shell = points within `w` cells of any face. Enumerated as: the 2w
 // y-planes at each end (all x, z), then inside them the 2w x-rows (all z), then the
 // 2w z-points of the remaining columns. Returns false past the end.

@@ -71,35 +71,36 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
     auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
-    const int hx0 = max(0, x0 - 2 + lx) * sx + cz, hx1 = min(nx - 1, x0 + TX + lx) * sx + cz;
-    const int hz0 = cx * sx + max(0, z0 - 2 + lz), hz1 = cx * sx + min(nz - 1, z0 + TZ + lz);
+    HaloMap<TX> H;
+    H.init(lx, lz, x0, z0, nx, nz, sx);
     const bool inx = ix >= 4 && ix < nx - 4, inz = iz >= 4 && iz < nz - 4;
-    float q[3][5];
+    float q[3][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        q[c][0] = ldy(curr + c * n3, y0 - 2);
-        q[c][1] = ldy(curr + c * n3, y0 - 1);
-        q[c][2] = ldy(curr + c * n3, y0);
-        q[c][3] = ldy(curr + c * n3, y0 + 1);
-    }
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int o = 0; o < 5; o++) q[c][o] = ldy(curr + c * n3, y0 - 2 + o);
+    float hx[3] = {0.f, 0.f, 0.f}, hz[3] = {0.f, 0.f, 0.f};  // halos of the next plane to stage
+    auto prefetch = [&](int y) {
+        const int pl = min(y, ny - 1) * sy;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (H.has_x) hx[c] = curr[c * n3 + pl + H.xsrc];
+            if (H.has_z) hz[c] = curr[c * n3 + pl + H.zsrc];
+        }
+    };
+    prefetch(y0);
     for (int iy = y0; iy < y1; iy++) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) q[c][4] = ldy(curr + c * n3, iy + 2);
+        for (int c = 0; c < 3; c++) q[c][5] = ldy(curr + c * n3, iy + 3);
         const int pl = iy * sy;
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float* __restrict__ u = curr + c * n3;
             su[c][lx + 2][lz + 2] = q[c][2];
-            if (lx < 2) {
-                su[c][lx][lz + 2] = u[pl + hx0];
-                su[c][TX + 2 + lx][lz + 2] = u[pl + hx1];
-            }
-            if (lz < 2) {
-                su[c][lx + 2][lz] = u[pl + hz0];
-                su[c][lx + 2][TZ + 2 + lz] = u[pl + hz1];
-            }
+            if (H.has_x) su[c][H.xrow][H.xcol] = hx[c];
+            if (H.has_z) su[c][H.zrow][H.zcol] = hz[c];
         }
+        prefetch(iy + 1);
         __syncthreads();
         if (valid) {
             const int p = pl + col;
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            q[c][0] = q[c][1]; q[c][1] = q[c][2]; q[c][2] = q[c][3]; q[c][3] = q[c][4];
+            q[c][0] = q[c][1]; q[c][1] = q[c][2]; q[c][2] = q[c][3]; q[c][3] = q[c][4]; q[c][4] = q[c][5];
         }
     }
 }
@@ -196,8 +197,8 @@ __global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
     const int col = cx * sx + cz;
     const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
     auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
-    const int hx0 = max(0, x0 - 2 + lx) * sx + cz, hx1 = min(nx - 1, x0 + TX + lx) * sx + cz;
-    const int hz0 = cx * sx + max(0, z0 - 2 + lz), hz1 = cx * sx + min(nz - 1, z0 + TZ + lz);
+    HaloMap<TX> H;
+    H.init(lx, lz, x0, z0, nx, nz, sx);
 
     // queue sources: u(3), b(NB), T_y, F_yx, F_yz
     const float* qsrc[NQ];
@@ -221,34 +222,41 @@ __global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
     psrc[PF + 2] = Ff + FX_ZY * n3;
     psrc[PF + 3] = Ff + FX_XZ * n3;
 
-    float q[NQ][5];
+    float q[NQ][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
 #pragma unroll
-    for (int k = 0; k < NQ; k++) {
-        q[k][0] = ldy(qsrc[k], y0 - 2);
-        q[k][1] = ldy(qsrc[k], y0 - 1);
-        q[k][2] = ldy(qsrc[k], y0);
-        q[k][3] = ldy(qsrc[k], y0 + 1);
-    }
+    for (int k = 0; k < NQ; k++)
+#pragma unroll
+        for (int o = 0; o < 5; o++) q[k][o] = ldy(qsrc[k], y0 - 2 + o);
     const float ih2[3] = {1.0f / (dx * dx), 1.0f / (dy * dy), 1.0f / (dz * dz)};
+    // values of the next plane to stage (centres of the flux planes, halos of all
+    // planes), loaded one iteration ahead of their use
+    float pc[NPL], hx[NPL], hz[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; k++) pc[k] = hx[k] = hz[k] = 0.f;
+    auto prefetch = [&](int y) {
+        const int pl = min(y, ny - 1) * sy;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            const float* __restrict__ f = psrc[k];
+            if (k >= 3 + NB) pc[k] = f[pl + col];
+            if (H.has_x) hx[k] = f[pl + H.xsrc];
+            if (H.has_z) hz[k] = f[pl + H.zsrc];
+        }
+    };
+    prefetch(y0);
 
     for (int iy = y0; iy < y1; iy++) {
 #pragma unroll
-        for (int k = 0; k < NQ; k++) q[k][4] = ldy(qsrc[k], iy + 2);
+        for (int k = 0; k < NQ; k++) q[k][5] = ldy(qsrc[k], iy + 3);
         const int pl = iy * sy;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
-            const float* __restrict__ f = psrc[k];
-            s[k][lx + 2][lz + 2] = (k < 3 + NB) ? q[k][2] : f[pl + col];
-            if (lx < 2) {
-                s[k][lx][lz + 2] = f[pl + hx0];
-                s[k][TX + 2 + lx][lz + 2] = f[pl + hx1];
-            }
-            if (lz < 2) {
-                s[k][lx + 2][lz] = f[pl + hz0];
-                s[k][lx + 2][TZ + 2 + lz] = f[pl + hz1];
-            }
+            s[k][lx + 2][lz + 2] = (k < 3 + NB) ? q[k][2] : pc[k];
+            if (H.has_x) s[k][H.xrow][H.xcol] = hx[k];
+            if (H.has_z) s[k][H.zrow][H.zcol] = hz[k];
         }
+        prefetch(iy + 1);
         __syncthreads();
         if (comp_xz && iy >= 6 && iy < ny - 6) {
             const int p = pl + col;
@@ -295,7 +303,7 @@ __global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
         }
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
-            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4];
+            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
         }
     }
 }
@@ -343,6 +351,7 @@ __device__ __forceinline__ float e_scale(const Geom& g, const int* bc, const ECo
     float sc = 0.f;
     bool any = false;
     // faces: top(z0) bottom(z1) left(x0) right(x1) front(y0) back(y1)
+#pragma unroll
     for (int f = 0; f < 6; f++) {
         if (bc[f] != 2) continue;
         const int ax = f < 2 ? 2 : (f < 4 ? 0 : 1);
@@ -367,33 +376,34 @@ __global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
     const Geom g = A.g;
     int ix, iy, iz;
     if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    const int c = blockIdx.y;  // displacement component of this thread
     constexpr int NT = VTI ? 3 : 1;
     const int n3 = g.n;
     const float* __restrict__ curr = A.curr;
     const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-    const float* bf[3] = {A.mu, VTI ? A.c66 : A.mu, VTI ? A.c11b : A.mu};
-    const float* T[3] = {A.aux, A.aux + (VTI ? n3 : 0), A.aux + (VTI ? 2 * n3 : 0)};
+    // selects instead of small arrays: c is a runtime value and indexed arrays would spill
+    auto BF = [&](int k) -> const float* { return k == 0 ? A.mu : (k == 1 ? (VTI ? A.c66 : A.mu) : (VTI ? A.c11b : A.mu)); };
+    auto IDX = [&](int a) { return a == 0 ? ix : (a == 1 ? iy : iz); };
     const float* F = A.aux + NT * n3;
     const int p = g.at(ix, iy, iz);
-    const int idx[3] = {ix, iy, iz};
-    auto D = [&](const float* f, int a) { return d1_point(f, p, idx[a], ax_len(g, a), ax_stride(g, a)); };
+    auto D = [&](const float* f, int a) { return d1_point(f, p, IDX(a), ax_len(g, a), ax_stride(g, a)); };
     const bool face = (iz == 0) | (iz == g.nz - 1) | (ix == 0) | (ix == g.nx - 1) | (iy == 0) | (iy == g.ny - 1);
     const float rho = M.rho[p];
-    for (int c = 0; c < 3; c++) {
+    {
         const float* u = curr + c * n3;
-        float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2_point(u, bf[d2_bfield<VTI>(c, 0)], p, ix, g.nx, g.sx);
-        acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2_point(u, bf[d2_bfield<VTI>(c, 1)], p, iy, g.ny, g.sy);
-        acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2_point(u, bf[d2_bfield<VTI>(c, 2)], p, iz, g.nz, 1);
+        float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2_point(u, BF(d2_bfield<VTI>(c, 0)), p, ix, g.nx, g.sx);
+        acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2_point(u, BF(d2_bfield<VTI>(c, 1)), p, iy, g.ny, g.sy);
+        acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2_point(u, BF(d2_bfield<VTI>(c, 2)), p, iz, g.nz, 1);
         if (c == 0) {
-            acc = acc + D(T[0], 0) / g.dx;
+            acc = acc + D(A.aux, 0) / g.dx;
             acc = acc + D(F + FX_YX * n3, 1) / g.dy;
             acc = acc + D(F + FX_ZX * n3, 2) / g.dy;  // spacing dy as in the reference
         } else if (c == 1) {
-            acc = acc + D(T[1], 1) / g.dy;
+            acc = acc + D(A.aux + (VTI ? n3 : 0), 1) / g.dy;
             acc = acc + D(F + FX_XY * n3, 0) / g.dx;
             acc = acc + D(F + FX_ZY * n3, 2) / g.dz;
         } else {
-            acc = acc + D(T[2], 2) / g.dz;
+            acc = acc + D(A.aux + (VTI ? 2 * n3 : 0), 2) / g.dz;
             acc = acc + D(F + FX_XZ * n3, 0) / g.dx;
             acc = acc + D(F + FX_YZ * n3, 1) / g.dy;
         }
@@ -401,13 +411,14 @@ __global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
         if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
         if (face) {
             // traction SATs of the faces through p (reference face order)
+#pragma unroll
             for (int f = 0; f < 6; f++) {
                 const int bc = A.bc[f];
                 if (bc != 1 && bc != 2) continue;
                 const int nax = f < 2 ? 2 : (f < 4 ? 0 : 1);
                 const bool hi = f & 1;
                 const int len = ax_len(g, nax);
-                if (!(hi ? idx[nax] == len - 1 : idx[nax] == 0)) continue;
+                if (!(hi ? IDX(nax) == len - 1 : IDX(nax) == 0)) continue;
                 const int t1 = nax == 2 ? 0 : (nax == 0 ? 1 : 0);
                 const int t2 = nax == 2 ? 1 : 2;
                 const float sg = hi ? 1.f : -1.f;
@@ -435,13 +446,15 @@ __global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
     }
 }
 
-// Source injection of one displacement component: next += taper(dt^2 inj / rho / scale);
-// mode 1: initial prev = 0.5 dt^2 inj(src[0]) / rho.
+// Source injection, all three displacement components (blockIdx.y):
+// next += taper(dt^2 inj / rho / scale); mode 1: initial prev = 0.5 dt^2 inj(src[0]) / rho.
 template <bool VTI>
-__global__ void elastic_inject(float* __restrict__ out, EArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
-                               int comp, int mode) {
+__global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
+                               int mode) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= I.npts * 27) return;
+    const int comp = blockIdx.y;
+    float* __restrict__ out = out3 + comp * A.g.n;
     const Geom g = A.g;
     const float dt = A.dt;
     const int r = t / 27, qq = t % 27;

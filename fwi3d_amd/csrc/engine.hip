@@ -311,7 +311,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         if (fwd_snap && it % sub == 0)
             HIP_OK(hipMemcpyAsync(snaps + (size_t)(it / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         if (!adj && E.npts > 0)
-            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it, 0);
+            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it);
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
@@ -324,7 +324,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     if (fwd_snap && (nt - 1) % sub == 0)
         HIP_OK(hipMemcpyAsync(snaps + (size_t)((nt - 1) / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     if (!adj && E.npts > 0)
-        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1, 0);
+        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1);
     if (adj && d_grad && sub > 0) grad_event(0, curr);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -395,12 +395,10 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int nbi = (I.npts * 27 + nthr - 1) / nthr;
     auto inject = [&](float* out, int it, int mode) {
         if (I.npts == 0) return;
-        for (int cc = 0; cc < 3; cc++) {
-            if (vti)
-                hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, A, I, d_src, nt, it, cc, mode);
-            else
-                hipLaunchKernelGGL(elastic_inject<false>, dim3(nbi), dim3(nthr), 0, c->stream, out + cc * n, A, I, d_src, nt, it, cc, mode);
-        }
+        if (vti)
+            hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi, 3), dim3(nthr), 0, c->stream, out, A, I, d_src, nt, it, mode);
+        else
+            hipLaunchKernelGGL(elastic_inject<false>, dim3(nbi, 3), dim3(nthr), 0, c->stream, out, A, I, d_src, nt, it, mode);
     };
     inject(prev, 0, 1);
 
@@ -424,8 +422,7 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
     auto extract = [&](const float* u, int it) {
         if (adj || E.npts == 0) return;
-        for (int cc = 0; cc < 3; cc++)
-            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128), dim3(128), 0, c->stream, u + cc * n, g, E, d_rcv, nt, it, cc);
+        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 3), dim3(128), 0, c->stream, u, g, E, d_rcv, nt, it);
     };
 
     for (int it = 0; it < nt - 1; it++) {
@@ -438,11 +435,11 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (vti) {
             timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
             timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<true>, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<true>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
         } else {
             timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
             timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<false>, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<false>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
