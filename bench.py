@@ -55,6 +55,33 @@ def cpu_baseline(shot, seconds=15.0):
                       f"(oracle FP32 C++/OpenMP, {t:.1f} s)"}
 
 
+def measured_traffic(n):
+    """HBM bytes per elastic time step (scaled FETCH_SIZE + WRITE_SIZE of the step kernels)
+    from the committed rocprofv3 --pmc summary of the same workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"elastic_{n}cube_step_traffic_pmc.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)["traffic_bytes_per_step"])
+    except (IndexError, KeyError, ValueError):
+        return None
+
+
+def copy_probe(torch, mib):
+    """A plain elementwise device copy of `mib` MiB, 5 launches: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch_read": n * 4, "copy_probe_bytes_per_launch_write": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +90,8 @@ def main():
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates the HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (shots would shard one per GPU); "
@@ -70,6 +99,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        copy_probe(torch, args.copy_probe)
+        return
     import fwi3d_amd as F
     from fwi3d_amd import workloads as W
 
@@ -128,8 +160,7 @@ def main():
         lat.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    step_ms, step_n = eng.profile_read(0)
-    flux_ms, flux_n = eng.profile_read(1)
+    prof = {name: eng.profile_read(k) for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"))}
     eng.profile(False)
 
     g = grad.cpu().numpy()
@@ -137,10 +168,11 @@ def main():
 
     updates = 2.0 * N * (nt - 1) * args.steps  # forward + adjoint cell-updates
     gcell = updates / elapsed / 1e9
-    t_step = step_ms / max(step_n, 1) / 1e3
-    t_flux = flux_ms / max(flux_n, 1) / 1e3
-    t_pair = t_step + t_flux
+    # one elastic time step = flux pre-pass + interior y-march + boundary shell launches
+    avg_us = {k: ms / max(n, 1) * 1e3 for k, (ms, n) in prof.items()}
+    t_pair = sum(avg_us.values()) / 1e6
     achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
+    traffic = measured_traffic(args.n)
 
     out = {
         "metric": METRIC,
@@ -168,11 +200,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "elastic time step = elastic_flux + elastic_step launches",
+            "traffic": traffic,
+            "traffic_source": f"profiles/elastic_{args.n}cube_step_traffic_pmc.md" if traffic else None,
+            "kernel": "elastic time step = elastic_flux + elastic_interior + elastic_boundary launches "
+                      "(HIP events on the engine stream)",
             "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
-            "avg_us": {"elastic_step": round(t_step * 1e6, 2), "elastic_flux": round(t_flux * 1e6, 2)},
-            "launches": {"elastic_step": step_n, "elastic_flux": flux_n},
+            "cells_per_step": N,
+            "avg_us": {k: round(v, 2) for k, v in avg_us.items()},
+            "launches": {k: n for k, (ms, n) in prof.items()},
         },
     }
     if not args.no_cpu_baseline:
