@@ -82,6 +82,87 @@ def copy_probe(torch, mib):
     print(json.dumps({"copy_probe_bytes_per_launch_read": n * 4, "copy_probe_bytes_per_launch_write": n * 4}), flush=True)
 
 
+def run_lbfgs(args, torch, F, W):
+    """Mixed-precision elastic FWI: 256^3 (or --n), nt=3000 (or --nt), layered model with
+    the anomaly as the target, anomaly-free starting model, snapshots every 4 steps in
+    --snap precision, --iters l-BFGS iterations (m=5, regular Wolfe) with FP64 dot
+    products; the gradient at the starting model is compared with the FP32-snapshot one."""
+    n, nt = args.n if args.n != 200 else 256, args.nt if args.nt != 2000 else 3000
+    true = W.config_b(n=n, nt=nt)
+    start = W.config_b(n=n, nt=nt, anomaly=False)
+    shape = true.shape
+    dev = torch.device("cuda:0")
+    eng = F.Engine(0)
+    geo = F.Shot(shape, true.d, (0.0, 0.0, 0.0), nt, true.dt, true.src_loc[0], true.rcv_loc)
+    opkw = dict(nmodels=3, dt=true.dt, sub=true.sub, mt=1, bc=true.bc, taper=true.taper,
+                taper_strength=true.taper_strength)
+    src = torch.from_numpy(np.ascontiguousarray(true.src[:, 0, :])).to(dev)
+    m_true = torch.from_numpy(np.ascontiguousarray(true.raw)).to(dev)
+    dobs = torch.zeros((3, geo.nrcv, nt), dtype=torch.float32, device=dev)
+    F.apply_forward_device(eng, F.op_param(**dict(opkw, sub=0)), geo, False, m_true.data_ptr(), src.data_ptr(),
+                           dobs.data_ptr())
+    del m_true
+    m = torch.from_numpy(np.ascontiguousarray(start.raw)).to(dev)
+    g = torch.zeros_like(m)
+    g64 = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+
+    def gradient(fmt):
+        g64.zero_()
+        f = F.fwi_shot_device(eng, F.op_param(**dict(opkw, snap_precision=fmt)), geo, m.data_ptr(), src.data_ptr(),
+                              dobs.data_ptr(), g64.data_ptr())
+        return f, g64.clone()
+
+    t0 = time.perf_counter()
+    _, g_ref = gradient("f32")
+    _, g_mp = gradient(args.snap)
+    t_two = time.perf_counter() - t0
+    err = [float(torch.linalg.vector_norm(g_mp[k] - g_ref[k]) / torch.linalg.vector_norm(g_ref[k])) for k in range(3)]
+    del g_ref, g_mp
+    evals = [0]
+
+    def objective(mp, gp):
+        g64.zero_()
+        f = F.fwi_shot_device(eng, F.op_param(**dict(opkw, snap_precision=args.snap)), geo, mp, src.data_ptr(),
+                              dobs.data_ptr(), g64.data_ptr())
+        g.copy_(g64.float())
+        torch.cuda.synchronize()
+        evals[0] += 1
+        print(f"# evaluation {evals[0]}: misfit {f:.6e}", file=sys.stderr, flush=True)
+        return f
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist, (it, fe, ge) = F.lbfgs_run(eng, m.numel(), m.data_ptr(), g.data_ptr(), objective, niter=args.iters)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    N = float(np.prod(shape))
+    out = {
+        "metric": METRIC,
+        "value": round(2.0 * N * (nt - 1) * fe / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": fe,
+        "warmup": 0,
+        "ms_per_step": round(elapsed / max(fe, 1) * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision elastic FWI, l-BFGS (m=5, regular Wolfe, FP64 dots), 1 shot, "
+                               f"{shape[2]}x{shape[1]}x{shape[0]}, nt={nt}, dt={true.dt}, sub={true.sub}, "
+                               f"{args.snap.upper()} snapshots with per-snapshot max-abs scale",
+                   "iterations": it, "function_evaluations": fe, "gradient_evaluations": ge},
+        "shot_gradients_per_s": round(fe / elapsed, 4),
+        "misfit_history": hist,
+        "misfit_decrease": round(1.0 - hist[-1] / hist[0], 6) if hist else None,
+        "snapshot_gradient_rel_l2_vs_fp32": {"vp": err[0], "vs": err[1], "rho": err[2]},
+        "two_gradients_s": round(t_two, 2),
+    }
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +171,10 @@ def main():
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
+    ap.add_argument("--workload", default="gradient", choices=("gradient", "lbfgs"),
+                    help="gradient: elastic shot gradient (default); lbfgs: 256^3 mixed-precision l-BFGS FWI")
+    ap.add_argument("--iters", type=int, default=10, help="l-BFGS iterations (--workload lbfgs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates the HBM traffic counters)")
     args = ap.parse_args()
@@ -105,6 +190,10 @@ def main():
     import fwi3d_amd as F
     from fwi3d_amd import workloads as W
 
+    if args.workload == "lbfgs":
+        run_lbfgs(args, torch, F, W)
+        return
+
     shot = W.config_b(n=args.n, nt=args.nt)
     obs = W.config_b(n=args.n, nt=args.nt, anomaly=False)
     shape = shot.shape
@@ -113,38 +202,31 @@ def main():
     nt = shot.nt
     dev = torch.device("cuda:0")
     eng = F.Engine(0)
-    inj = F.Injector(shot.src_kind, shot.src_loc, shape, shot.d)
-    ext = F.Injector(0, shot.rcv_loc, shape, shot.d)
-    par = F.make_param(nt, shot.dt, sub=shot.sub, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
-    par0 = F.make_param(nt, shot.dt, sub=0, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
+    geo = F.Shot(shape, shot.d, (0.0, 0.0, 0.0), nt, shot.dt, shot.src_loc[0], shot.rcv_loc)
+    # the reference's param for this shot: raw (vp, vs, rho), explosive moment tensor,
+    # propagation dt = data dt (sinc resampling still runs), imaging every 4 steps
+    opkw = dict(nmodels=3, dt=shot.dt, sub=shot.sub, mt=1, bc=shot.bc, taper=shot.taper,
+                taper_strength=shot.taper_strength, snap_precision=args.snap)
 
-    model = torch.from_numpy(shot.model).to(dev)
-    model_obs = torch.from_numpy(obs.model).to(dev)
-    src = torch.from_numpy(np.ascontiguousarray(shot.src)).to(dev)
-    rcv = torch.zeros((3, ext.npts, nt), dtype=torch.float32, device=dev)
-    dobs = torch.zeros_like(rcv)
-    res = torch.zeros_like(rcv)
+    model = torch.from_numpy(np.ascontiguousarray(shot.raw)).to(dev)
+    model_obs = torch.from_numpy(np.ascontiguousarray(obs.raw)).to(dev)
+    src = torch.from_numpy(np.ascontiguousarray(shot.src[:, 0, :])).to(dev)  # [6][nt] wavelets
+    dobs = torch.zeros((3, geo.nrcv, nt), dtype=torch.float32, device=dev)
     grad = torch.zeros((3, ny, nx, nz), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    # observed data (not timed)
-    eng.propagate_device("elastic", model_obs.data_ptr(), src.data_ptr(), dobs.data_ptr(), inj, ext, par0, shape, shot.d)
+    # observed data: the anomaly-free model (not timed)
+    F.apply_forward_device(eng, F.op_param(**dict(opkw, sub=0)), geo, False, model_obs.data_ptr(), src.data_ptr(),
+                           dobs.data_ptr())
 
     misfits = []
 
     def shot_gradient():
-        eng.propagate_device("elastic", model.data_ptr(), src.data_ptr(), rcv.data_ptr(), inj, ext, par, shape, shot.d)
-        torch.cuda.synchronize()
-        r = rcv - dobs
-        # trapezoid misfit 0.5*dt*(sum r^2 - (r_0^2 + r_{nt-1}^2)/2) in FP64 (optimization.hpp:410-424)
-        r64 = r.double()
-        misfits.append(float(0.5 * shot.dt * ((r64 * r64).sum() - 0.5 * (r64[..., 0] ** 2 + r64[..., -1] ** 2).sum())))
-        # adjoint source: -alpha * time-reversed residual (Ht then Ht^-1 cancel; we_op.cpp:1001-1003)
-        res.copy_(-torch.flip(r, dims=[2]))
+        # one shot of nlls_fwi::compute_res_and_grad: forward with snapshots, residual and
+        # FP64 misfit, adjoint with the imaging condition, quadrature and chain rule
         grad.zero_()
-        torch.cuda.synchronize()
-        eng.propagate_device("elastic", model.data_ptr(), res.data_ptr(), None, ext, None, par, shape, shot.d,
-                             adj=True, grad64_ptr=grad.data_ptr())
+        misfits.append(F.fwi_shot_device(eng, F.op_param(**opkw), geo, model.data_ptr(), src.data_ptr(),
+                                         dobs.data_ptr(), grad.data_ptr()))
 
     for _ in range(args.warmup):
         shot_gradient()
@@ -160,7 +242,8 @@ def main():
         lat.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = {name: eng.profile_read(k) for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"))}
+    prof = {name: eng.profile_read(k)
+            for k, name in ((0, "elastic_step"), (1, "elastic_flux"), (2, "elastic_faces"), (3, "elastic_zslab"))}
     eng.profile(False)
 
     g = grad.cpu().numpy()
@@ -168,7 +251,7 @@ def main():
 
     updates = 2.0 * N * (nt - 1) * args.steps  # forward + adjoint cell-updates
     gcell = updates / elapsed / 1e9
-    # one elastic time step = flux pre-pass + interior y-march + boundary shell launches
+    # one elastic time step = flux pre-pass + y-march step + z slabs + face SATs launches
     avg_us = {k: ms / max(n, 1) * 1e3 for k, (ms, n) in prof.items()}
     t_pair = sum(avg_us.values()) / 1e6
     achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
@@ -188,8 +271,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"elastic isotropic forward + adjoint gradient, 1 shot, {nz}x{nx}x{ny}, nt={nt}, "
-                               f"dt={shot.dt}, sub={shot.sub}, FP32 snapshots, FP64 gradient accumulation",
-                   "nz": nz, "nx": nx, "ny": ny, "nt": nt, "receivers": int(ext.npts) * 3},
+                               f"dt={shot.dt}, sub={shot.sub}, {args.snap.upper()} snapshots, FP64 misfit and gradient "
+                               f"accumulation (fwi3d_fwi_shot_device)",
+                   "nz": nz, "nx": nx, "ny": ny, "nt": nt, "receivers": int(geo.nrcv) * 3},
         "shot_gradients_per_s": round(args.steps / elapsed, 4),
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "misfit": misfits[-1] if misfits else None,
@@ -202,7 +286,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": f"profiles/elastic_{args.n}cube_step_traffic_pmc.md" if traffic else None,
-            "kernel": "elastic time step = elastic_flux + elastic_interior + elastic_boundary launches "
+            "kernel": "elastic time step = elastic_flux + elastic_step + elastic_zslab + elastic_faces launches "
                       "(HIP events on the engine stream)",
             "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
             "cells_per_step": N,

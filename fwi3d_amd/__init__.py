@@ -24,6 +24,9 @@ SNAP = {"f32": 0, "bf16": 1, "f16": 2}
 EXPORTS = (
     "fwi3d_create", "fwi3d_destroy", "fwi3d_last_error", "fwi3d_injector_tables",
     "fwi3d_propagate", "fwi3d_propagate_device", "fwi3d_profile", "fwi3d_profile_read",
+    "fwi3d_op_param_defaults", "fwi3d_analyze_model", "fwi3d_apply_forward", "fwi3d_apply_jacobianT",
+    "fwi3d_apply_forward_device", "fwi3d_apply_jacobianT_device", "fwi3d_fwi_shot_device",
+    "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run",
 )
 
 
@@ -42,6 +45,37 @@ class InjectorStruct(C.Structure):
                 ("xw", C.c_void_p), ("yw", C.c_void_p), ("zw", C.c_void_p)]
 
 
+class OpParam(C.Structure):
+    """fwi3d_op_param (include/fwi3d_hip.h): the reference's param fields read by the
+    shot operator; dt, courant, sub, nt, vmin, vmax are updated by each call."""
+    _fields_ = [("nmodels", C.c_int), ("dt", C.c_float), ("courant", C.c_float), ("sub", C.c_int),
+                ("nt", C.c_int), ("vmin", C.c_float), ("vmax", C.c_float), ("resampling", C.c_int),
+                ("sinc_half_length", C.c_int), ("bc", C.c_int * 6), ("taper", C.c_int * 6),
+                ("taper_strength", C.c_float), ("free_surface_stiffness", C.c_float), ("mt", C.c_int),
+                ("seismotype", C.c_int), ("soft_clip", C.c_int)] + \
+               [(k, C.c_float) for k in ("vpmin", "vpmax", "vsmin", "vsmax", "rhomin", "rhomax", "deltamin",
+                                         "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")] + \
+               [("snap_precision", C.c_int)]
+
+
+class ShotStruct(C.Structure):
+    """fwi3d_shot (include/fwi3d_hip.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+               [(k, C.c_float) for k in ("dx", "dy", "dz", "ox", "oy", "oz")] + \
+               [("nt_data", C.c_int), ("dt_data", C.c_float), ("sxyz", C.c_void_p), ("nrcv", C.c_int),
+                ("rxyz", C.c_void_p)]
+
+
+class LbfgsParam(C.Structure):
+    """fwi3d_lbfgs_param (include/fwi3d_hip.h)."""
+    _fields_ = [("niter", C.c_int), ("m", C.c_int), ("max_trial", C.c_int), ("threshold", C.c_double),
+                ("c1", C.c_double), ("c2", C.c_double), ("a0", C.c_double), ("a1", C.c_double),
+                ("step_max", C.c_double), ("version", C.c_int)]
+
+
+# int (*)(void *user, const float *d_model, float *d_grad, double *f)
+OBJECTIVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double))
+
 _lib = None
 
 
@@ -58,6 +92,12 @@ def lib():
         L.fwi3d_last_error.argtypes = [C.c_void_p]
         L.fwi3d_profile.argtypes = [C.c_void_p, C.c_int]
         L.fwi3d_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        L.fwi3d_op_param_defaults.restype = None
+        L.fwi3d_op_param_defaults.argtypes = [C.POINTER(OpParam)]
+        L.fwi3d_lbfgs_param_defaults.restype = None
+        L.fwi3d_lbfgs_param_defaults.argtypes = [C.POINTER(LbfgsParam)]
+        L.fwi3d_lbfgs_run.argtypes = [C.c_void_p, C.POINTER(LbfgsParam), C.c_longlong, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, OBJECTIVE, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -184,3 +224,136 @@ class Engine:
         n = C.c_longlong()
         self._check(lib().fwi3d_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)), "fwi3d_profile_read")
         return ms.value, n.value
+
+
+def op_param(**kw):
+    """fwi3d_op_param with the reference's defaults (fwi3d_op_param_defaults) and the
+    given fields; `bc`/`taper` take 6-tuples, `resampling` "sinc"/"linear" or 0/1."""
+    p = OpParam()
+    lib().fwi3d_op_param_defaults(C.byref(p))
+    for k, v in kw.items():
+        if k in ("bc", "taper"):
+            setattr(p, k, (C.c_int * 6)(*v))
+        elif k == "resampling" and isinstance(v, str):
+            p.resampling = 1 if v == "linear" else 0
+        elif k == "snap_precision" and isinstance(v, str):
+            p.snap_precision = SNAP[v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Shot:
+    """fwi3d_shot with its position arrays kept alive."""
+
+    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz):
+        ny, nx, nz = shape
+        self.shape = tuple(shape)
+        self.sxyz = np.ascontiguousarray(np.asarray(sxyz, np.float32).reshape(3))
+        self.rxyz = np.ascontiguousarray(np.asarray(rxyz, np.float32).reshape(-1, 3))
+        self.nrcv = int(self.rxyz.shape[0])
+        self.nt_data = int(nt_data)
+        self.c = ShotStruct(nx, ny, nz, d[0], d[1], d[2], o[0], o[1], o[2], int(nt_data), float(dt_data),
+                            self.sxyz.ctypes.data, self.nrcv, self.rxyz.ctypes.data)
+
+
+def analyze_model(par, model, shot):
+    """analyzeModel on a copy of the raw model: returns the clipped copy; par gets the
+    time axis (fwi3d_analyze_model, host code)."""
+    m = np.ascontiguousarray(model, dtype=np.float32).copy()
+    rc = lib().fwi3d_analyze_model(C.byref(par), C.c_void_p(m.ctypes.data), C.byref(shot.c))
+    if rc != 0:
+        raise ValueError(f"fwi3d_analyze_model failed ({rc})")
+    return m
+
+
+class WeOp:
+    """Nonlinear wave-equation operator of one shot, the interface of nl_we_op_e /
+    nl_we_op_a / nl_we_op_vti (src/include/we_op.hpp:13-170): the physics follows
+    par.nmodels (2 acoustic, 3 elastic, 6 VTI). Snapshots of apply_forward stay on the
+    engine for the next apply_jacobianT, as on the reference's operator instance."""
+
+    def __init__(self, engine, shot, src, par):
+        self.eng = engine
+        self.shot = shot
+        self.src = np.ascontiguousarray(src, dtype=np.float32)
+        self.par = par
+
+    @property
+    def nrcomp(self):
+        return 1 if self.par.nmodels == 2 else 3
+
+    def apply_forward(self, add, model, data=None):
+        """model [nmodels][ny][nx][nz] raw parameters -> data [nrcomp][nrcv][nt_data]."""
+        model = np.ascontiguousarray(model, dtype=np.float32)
+        if data is None:
+            data = np.zeros((self.nrcomp, self.shot.nrcv, self.shot.nt_data), np.float32)
+        assert data.dtype == np.float32 and data.flags.c_contiguous
+        rc = lib().fwi3d_apply_forward(self.eng._h, C.byref(self.par), C.byref(self.shot.c), int(bool(add)),
+                                       _vp(model), _vp(self.src), _vp(data))
+        self.eng._check(rc, "fwi3d_apply_forward")
+        return data
+
+    def apply_jacobianT(self, add, grad, model0, data):
+        """data residual -> grad [min(3,nmodels)][ny][nx][nz] (float32, in place)."""
+        model0 = np.ascontiguousarray(model0, dtype=np.float32)
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        assert grad.dtype == np.float32 and grad.flags.c_contiguous
+        rc = lib().fwi3d_apply_jacobianT(self.eng._h, C.byref(self.par), C.byref(self.shot.c), int(bool(add)),
+                                         _vp(grad), _vp(model0), _vp(data))
+        self.eng._check(rc, "fwi3d_apply_jacobianT")
+        return grad
+
+
+def apply_forward_device(engine, par, shot, add, model_ptr, src_ptr, data_ptr):
+    rc = lib().fwi3d_apply_forward_device(engine._h, C.byref(par), C.byref(shot.c), int(bool(add)),
+                                          C.c_void_p(model_ptr), C.c_void_p(src_ptr), C.c_void_p(data_ptr))
+    engine._check(rc, "fwi3d_apply_forward_device")
+
+
+def apply_jacobianT_device(engine, par, shot, add, grad64_ptr, model_ptr, data_ptr):
+    rc = lib().fwi3d_apply_jacobianT_device(engine._h, C.byref(par), C.byref(shot.c), int(bool(add)),
+                                            C.c_void_p(grad64_ptr), C.c_void_p(model_ptr), C.c_void_p(data_ptr))
+    engine._check(rc, "fwi3d_apply_jacobianT_device")
+
+
+def fwi_shot_device(engine, par, shot, model_ptr, src_ptr, obs_ptr, grad64_ptr, resid_ptr=None):
+    """One FWI shot on device buffers: returns the misfit contribution (FP64); the
+    gradient accumulates into grad64 [min(3,nmodels)][ny][nx][nz]."""
+    mis = C.c_double()
+    rc = lib().fwi3d_fwi_shot_device(engine._h, C.byref(par), C.byref(shot.c), C.c_void_p(model_ptr),
+                                     C.c_void_p(src_ptr), C.c_void_p(obs_ptr),
+                                     C.c_void_p(resid_ptr) if resid_ptr else None, C.c_void_p(grad64_ptr),
+                                     C.byref(mis))
+    engine._check(rc, "fwi3d_fwi_shot_device")
+    return mis.value
+
+
+def lbfgs_run(engine, n, model_ptr, grad_ptr, objective, niter, h0_ptr=None, **kw):
+    """l-BFGS on device vectors (fwi3d_lbfgs_run). `objective(model_ptr, grad_ptr) -> f`
+    evaluates the misfit and writes the gradient into the buffer at grad_ptr (the same
+    buffers as given here). Returns (f history, (iterations, feval, geval))."""
+    p = LbfgsParam()
+    lib().fwi3d_lbfgs_param_defaults(C.byref(p))
+    p.niter = int(niter)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    err = []
+
+    def cb(user, m, g, f):
+        try:
+            f[0] = float(objective(m, g))
+            return 0
+        except Exception as e:  # reported after the run
+            err.append(e)
+            return 1
+
+    fn = OBJECTIVE(cb)
+    hist = (C.c_double * (p.niter + 1))()
+    cnt = (C.c_int * 3)()
+    rc = lib().fwi3d_lbfgs_run(engine._h, C.byref(p), int(n), C.c_void_p(model_ptr), C.c_void_p(grad_ptr),
+                               C.c_void_p(h0_ptr) if h0_ptr else None, fn, None, hist, cnt)
+    if err:
+        raise err[0]
+    engine._check(rc, "fwi3d_lbfgs_run")
+    return list(hist)[: cnt[0] + 1], (cnt[0], cnt[1], cnt[2])

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfwi3d_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", "engine.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "operator.hip", "lbfgs.hip")]
 ARCH = os.environ.get("FWI3D_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

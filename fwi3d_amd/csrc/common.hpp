@@ -5,8 +5,13 @@
 // All kernels of the engine live in one translation unit (engine.hip includes
 // this header once), so the __constant__ tables below are defined exactly once.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <utility>
+
+#include "sbp_tables.hpp"
 
 namespace fwi3d {
 
@@ -108,10 +113,114 @@ __device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, in
     return d1_closure(u, p, i, n, s);
 }
 
+// ------------------------------------------------------------------------------------
+// Closure rows with compile-time coefficients: row R keeps only its non-zero entries
+// of the sparse tables, so a wave whose lanes sit on different rows runs each row's
+// ~22 FMAs under its own mask instead of loading the coefficient table per lane.
+// u(j) / b(k) return the field / coefficient at the j-th / k-th point from the face.
+template <int E>
+struct D2Term {
+    static constexpr int r = kD2Closure[E].r, j = kD2Closure[E].j, k = kD2Closure[E].k;
+    static constexpr float w = (float)((double)kD2Closure[E].num / (double)kD2Closure[E].den);
+};
+template <int R, int E, class U, class B>
+__device__ __forceinline__ float d2_term(const U& u, const B& b) {
+    if constexpr (D2Term<E>::r == R) return (D2Term<E>::w * b(D2Term<E>::k)) * u(D2Term<E>::j);
+    else return 0.f;
+}
+template <int R, class U, class B, int... E>
+__device__ __forceinline__ float d2_row_impl(const U& u, const B& b, std::integer_sequence<int, E...>) {
+    return (0.f + ... + d2_term<R, E>(u, b));
+}
+// D2(b) (times h^2) of closure row r in [0, 6).
+template <class U, class B>
+__device__ __forceinline__ float d2_rows(int r, const U& u, const B& b) {
+    using Seq = std::make_integer_sequence<int, 132>;
+    switch (r) {
+        case 0: return d2_row_impl<0>(u, b, Seq{});
+        case 1: return d2_row_impl<1>(u, b, Seq{});
+        case 2: return d2_row_impl<2>(u, b, Seq{});
+        case 3: return d2_row_impl<3>(u, b, Seq{});
+        case 4: return d2_row_impl<4>(u, b, Seq{});
+        default: return d2_row_impl<5>(u, b, Seq{});
+    }
+}
+template <int R, int J, class U>
+__device__ __forceinline__ float d1_term(const U& u) {
+    constexpr float w = (float)kD1Closure[R][J];
+    if constexpr (kD1Closure[R][J] != 0.0) return w * u(J);
+    else return 0.f;
+}
+template <int R, class U>
+__device__ __forceinline__ float d1_row(const U& u) {
+    return d1_term<R, 0>(u) + d1_term<R, 1>(u) + d1_term<R, 2>(u) + d1_term<R, 3>(u) + d1_term<R, 4>(u) + d1_term<R, 5>(u);
+}
+// D1 (times h) of closure row r in [0, 4), counted from the low face (negate for the
+// high face, where u(j) runs inward from the last point).
+template <class U>
+__device__ __forceinline__ float d1_rows(int r, const U& u) {
+    switch (r) {
+        case 0: return d1_row<0>(u);
+        case 1: return d1_row<1>(u);
+        case 2: return d1_row<2>(u);
+        default: return d1_row<3>(u);
+    }
+}
+
+// Closure rows read from global memory, out of line: the y-marching kernels take
+// these wave-uniform branches on a few x rows and y planes only, and keeping the
+// code out of the loop body keeps its register allocation at the interior path's.
+// base = offset of the face point, st = +/- stride inward.
+__device__ __attribute__((noinline)) float d2_rows_g(const float* __restrict__ u, const float* __restrict__ b, int base,
+                                                     int st, int r) {
+    return d2_rows(r, [&](int j) { return u[base + j * st]; }, [&](int k) { return b[base + k * st]; });
+}
+__device__ __attribute__((noinline)) float d1_rows_g(const float* __restrict__ f, int base, int st, int r) {
+    return d1_rows(r, [&](int j) { return f[base + j * st]; });
+}
+
 // Outward boundary derivative S (times h) at a face point p, `inward` = +/- stride.
 __device__ __forceinline__ float sat_s(const float* __restrict__ u, int p, int inward) {
     return c_sat_s[0] * u[p] + c_sat_s[1] * u[p + inward] + c_sat_s[2] * u[p + 2 * inward] + c_sat_s[3] * u[p + 3 * inward];
 }
+
+// D1 (times h) at axis index i through an accessor f(offset) (snapshot views).
+template <class F>
+__device__ __forceinline__ float d1_acc(const F& f, int p, int i, int n, int s) {
+    if (i >= 4 && i < n - 4) return D1C1 * (f(p + s) - f(p - s)) + D1C2 * (f(p + 2 * s) - f(p - 2 * s));
+    const bool lo = i < 4;
+    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
+    const int step = lo ? s : -s;
+    const float v = d1_rows(lo ? i : n - 1 - i, [&](int j) { return f(base + j * step); });
+    return lo ? v : -v;
+}
+
+// ------------------------------------------------------------------------------------
+// Snapshot store. FP32, or 16-bit codes (BF16 / FP16) of u / s with one scale per
+// snapshot, s = max|u| / 2^15: the largest value maps to 2^15, inside FP16's range,
+// and small values stay normal down to max|u| * 2^-29. The max is reduced on the
+// device as float bits (non-negative floats order like unsigned integers).
+enum SnapFmt { SNAP_F32 = 0, SNAP_BF16 = 1, SNAP_F16 = 2 };
+constexpr float kSnapTop = 32768.f;
+
+__device__ __forceinline__ unsigned short bf16_rne(float v) {
+    unsigned u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <int FMT>
+struct SnapView {
+    const void* p;           // slot base (null: a slot past the store, reads as zero)
+    const unsigned* maxbits;  // the slot's max|u| as float bits (16-bit formats)
+    __device__ float scale() const { return FMT == SNAP_F32 ? 1.f : __uint_as_float(*maxbits) / kSnapTop; }
+    __device__ float code(int i) const {
+        if (FMT == SNAP_F32) return ((const float*)p)[i];
+        if (FMT == SNAP_BF16) return bf16_val(((const unsigned short*)p)[i]);
+        return __half2float(((const __half*)p)[i]);
+    }
+};
 
 // Per-step boundary description shared by the physics kernels.
 struct Bound {
@@ -196,6 +305,16 @@ __device__ __forceinline__ bool shell_point(int t, int w, int nx, int ny, int nz
     return false;
 }
 
+// The two 6-deep z slabs (z < 6 and z >= nz-6) of every (x, y) column, 12 points per
+// column with consecutive threads on consecutive z. Returns false past the end.
+__device__ __forceinline__ bool zslab_point(int t, int nx, int ny, int nz, int& ix, int& iy, int& iz) {
+    const int col = t / 12, k = t - col * 12;
+    if (col >= nx * ny) return false;
+    iy = col / nx;
+    ix = col - iy * nx;
+    iz = k < 6 ? k : nz - 12 + k;
+    return true;
+}
 __host__ __device__ inline int shell_count(int w, int nx, int ny, int nz) {
     return 2 * w * nx * nz + (ny - 2 * w) * 2 * w * nz + (ny - 2 * w) * (nx - 2 * w) * 2 * w;
 }

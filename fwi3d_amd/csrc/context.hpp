@@ -1,0 +1,78 @@
+// Engine context and host-side helpers shared by the translation units of the
+// library (engine.hip: propagation; operator.hip: the shot operator around it).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fwi3d_hip.h"
+
+namespace fwi3d {
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIP_OK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            throw Err{FWI3D_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)};             \
+    } while (0)
+
+#define REQUIRE(cond, text)                                    \
+    do {                                                       \
+        if (!(cond)) throw Err{FWI3D_EINVAL, std::string(text)}; \
+    } while (0)
+
+// Growable device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t nb) {
+        if (nb <= bytes) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) throw Err{FWI3D_ENOMEM, "hipMalloc(" + std::to_string(nb) + " bytes) failed"};
+        bytes = nb;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+}  // namespace fwi3d
+
+struct fwi3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    fwi3d::DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;
+    // shot operator (operator.hip): converted model, reduction scalars, traces at the
+    // propagation rate, raw FP64 gradient, synthetic data + misfit of an FWI shot
+    fwi3d::DevBuf opmodel, opscal, optrace, opgrad, opsyn;
+    // l-BFGS (lbfgs.hip): curvature pairs, directions, backups, reduction scratch
+    fwi3d::DevBuf opt;
+    // snapshot-store state (the per-instance state of the reference operator)
+    fwi3d::DevBuf snapsc;  // per-slot max|u| (float bits) of 16-bit snapshot slots
+    int snap_slots = 0, snap_ncomp = 0, snap_nt = 0, snap_sub = 0, snap_physics = -1, snap_fmt = 0;
+    long long snap_n = 0;
+    bool snap_valid = false;
+    int snap_dims[3] = {0, 0, 0};
+    // profiling: [0] time-step kernel (z in [6, nz-6)), [1] flux pre-pass (elastic/VTI),
+    // [2] face SATs, [3] z-slab points
+    bool prof = false;
+    double prof_ms[4] = {0, 0, 0, 0};
+    long long prof_n[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    std::vector<int> ev_kind;
+};

@@ -1,5 +1,6 @@
 // Engine: context, device-resident state and the time loop of propagate(), behind
-// the C ABI of include/fwi3d_hip.h. All kernels are in this translation unit.
+// the C ABI of include/fwi3d_hip.h. The shot operator around propagate() is in
+// operator.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -7,56 +8,20 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fwi3d_hip.h"
 #include "acoustic.hpp"
+#include "context.hpp"
 #include "common.hpp"
 #include "elastic.hpp"
 #include "sbp_tables.hpp"
+#include "snapshots.hpp"
 
 using namespace fwi3d;
 
 namespace {
-
-struct Err {
-    int code;
-    std::string msg;
-};
-
-#define HIP_OK(expr)                                                                                 \
-    do {                                                                                             \
-        hipError_t _e = (expr);                                                                      \
-        if (_e != hipSuccess)                                                                        \
-            throw Err{FWI3D_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)};             \
-    } while (0)
-
-#define REQUIRE(cond, text)                                    \
-    do {                                                       \
-        if (!(cond)) throw Err{FWI3D_EINVAL, std::string(text)}; \
-    } while (0)
-
-// Growable device allocation.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t nb) {
-        if (nb <= bytes) return;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, nb);
-        if (e != hipSuccess) throw Err{FWI3D_ENOMEM, "hipMalloc(" + std::to_string(nb) + " bytes) failed"};
-        bytes = nb;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const { return (T*)p; }
-};
 
 bool g_tables_loaded[64] = {false};
 
@@ -77,26 +42,6 @@ void load_tables(int dev) {
 }
 
 }  // namespace
-
-struct fwi3d_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;
-    // snapshot-store state (the per-instance state of the reference operator)
-    int snap_slots = 0, snap_ncomp = 0, snap_nt = 0, snap_sub = 0, snap_physics = -1;
-    long long snap_n = 0;
-    bool snap_valid = false;
-    int snap_dims[3] = {0, 0, 0};
-    // profiling: [0] interior time-step kernel, [1] flux pre-pass (elastic/VTI),
-    // [2] boundary-shell time-step kernel
-    bool prof = false;
-    double prof_ms[3] = {0, 0, 0};
-    long long prof_n[3] = {0, 0, 0};
-    std::vector<hipEvent_t> ev;
-    size_t ev_used = 0;
-    std::vector<int> ev_kind;
-};
 
 namespace {
 
@@ -230,14 +175,73 @@ Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
     return Tapers{dtz, dtx, dty};
 }
 
-// Tiling of the y-marching kernels: 64(z) x TX(x) column tiles, y in chunks.
+// Tiling of the y-marching kernels: 64(z) x TX(x) column tiles over z in [zlo, zhi),
+// y in chunks.
 template <class Args>
-int tile_grid(Args& A, const Geom& g, int TX) {
-    A.ntz = (g.nz + 63) / 64;
+int tile_grid(Args& A, const Geom& g, int TX, int zlo, int zhi) {
+    A.zlo = zlo;
+    A.zhi = zhi;
+    A.ntz = (zhi - zlo + 63) / 64;
     A.ntx = (g.nx + TX - 1) / TX;
     A.ychunk = std::max(4, std::min(g.ny, 16));
     A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;
+}
+
+// ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero
+void snap_begin(fwi3d_ctx* c, int physics, int ncomp, const fwi3d_param* par, const Geom& g) {
+    const int slots = 1 + par->nt / par->sub;
+    const int esz = par->snap_precision == FWI3D_SNAP_F32 ? 4 : 2;
+    const size_t bytes = (size_t)slots * ncomp * g.n * esz;
+    c->snaps.ensure(bytes);
+    c->snapsc.ensure((size_t)slots * sizeof(unsigned));
+    HIP_OK(hipMemsetAsync(c->snaps.p, 0, bytes, c->stream));
+    HIP_OK(hipMemsetAsync(c->snapsc.p, 0, (size_t)slots * sizeof(unsigned), c->stream));
+    c->snap_slots = slots; c->snap_ncomp = ncomp; c->snap_n = g.n; c->snap_nt = par->nt; c->snap_sub = par->sub;
+    c->snap_physics = physics; c->snap_fmt = par->snap_precision;
+    c->snap_dims[0] = g.nx; c->snap_dims[1] = g.ny; c->snap_dims[2] = g.nz;
+    c->snap_valid = false;
+}
+
+void snap_check(const fwi3d_ctx* c, int physics, const fwi3d_param* par, const Geom& g) {
+    REQUIRE(c->snap_valid && c->snap_physics == physics && c->snap_nt == par->nt && c->snap_sub == par->sub &&
+                c->snap_dims[0] == g.nx && c->snap_dims[1] == g.ny && c->snap_dims[2] == g.nz,
+            "adjoint propagation needs the snapshots of a matching forward propagation on this context");
+    REQUIRE(c->snap_fmt == par->snap_precision, "adjoint snap_precision differs from the forward propagation's");
+}
+
+size_t snap_slot_elems(const fwi3d_ctx* c) { return (size_t)c->snap_ncomp * c->snap_n; }
+
+void snap_put(fwi3d_ctx* c, int k, const float* u) {
+    const size_t ne = snap_slot_elems(c);
+    unsigned* mb = c->snapsc.as<unsigned>() + k;
+    if (c->snap_fmt == FWI3D_SNAP_F32) {
+        HIP_OK(hipMemcpyAsync(c->snaps.as<float>() + (size_t)k * ne, u, ne * sizeof(float), hipMemcpyDeviceToDevice,
+                              c->stream));
+        return;
+    }
+    void* dst = c->snaps.as<char>() + (size_t)k * ne * 2;
+    const unsigned grid = (unsigned)std::min<size_t>(2048, (ne / 4 + 255) / 256 + 1);
+    hipLaunchKernelGGL(snap_maxabs, dim3(grid), dim3(256), 0, c->stream, u, mb, (long long)ne);
+    if (c->snap_fmt == FWI3D_SNAP_BF16)
+        hipLaunchKernelGGL(snap_store<SNAP_BF16>, dim3(2048), dim3(256), 0, c->stream, u, dst, (const unsigned*)mb, (long long)ne);
+    else
+        hipLaunchKernelGGL(snap_store<SNAP_F16>, dim3(2048), dim3(256), 0, c->stream, u, dst, (const unsigned*)mb, (long long)ne);
+}
+
+template <int FMT>
+SnapView<FMT> snap_view(const fwi3d_ctx* c, int k) {
+    if (k < 0 || k >= c->snap_slots) return SnapView<FMT>{nullptr, nullptr};
+    const size_t esz = FMT == SNAP_F32 ? 4 : 2;
+    return SnapView<FMT>{c->snaps.as<char>() + (size_t)k * snap_slot_elems(c) * esz, c->snapsc.as<unsigned>() + k};
+}
+
+// Call f(std::integral_constant<int, FMT>) for the context's snapshot format.
+template <class Fn>
+void with_snap_fmt(const fwi3d_ctx* c, Fn&& f) {
+    if (c->snap_fmt == FWI3D_SNAP_BF16) f(std::integral_constant<int, SNAP_BF16>{});
+    else if (c->snap_fmt == FWI3D_SNAP_F16) f(std::integral_constant<int, SNAP_F16>{});
+    else f(std::integral_constant<int, SNAP_F32>{});
 }
 
 void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const float* d_src, float* d_rcv,
@@ -260,20 +264,8 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     const int nt = par->nt, sub = par->sub;
     const float dt = par->dt;
     const bool fwd_snap = !adj && sub > 0;
-    if (fwd_snap) {
-        const int slots = 1 + nt / sub;
-        c->snaps.ensure((size_t)slots * n * sizeof(float));
-        HIP_OK(hipMemsetAsync(c->snaps.p, 0, (size_t)slots * n * sizeof(float), c->stream));
-        c->snap_slots = slots; c->snap_ncomp = 1; c->snap_n = n; c->snap_nt = nt; c->snap_sub = sub;
-        c->snap_physics = FWI3D_ACOUSTIC; c->snap_dims[0] = g.nx; c->snap_dims[1] = g.ny; c->snap_dims[2] = g.nz;
-        c->snap_valid = false;
-    }
-    if (adj && d_grad && sub > 0) {
-        REQUIRE(c->snap_valid && c->snap_physics == FWI3D_ACOUSTIC && c->snap_nt == nt && c->snap_sub == sub &&
-                    c->snap_dims[0] == g.nx && c->snap_dims[1] == g.ny && c->snap_dims[2] == g.nz,
-                "adjoint propagation needs the snapshots of a matching forward propagation on this context");
-    }
-    float* snaps = c->snaps.as<float>();
+    if (fwd_snap) snap_begin(c, FWI3D_ACOUSTIC, 1, par, g);
+    if (adj && d_grad && sub > 0) snap_check(c, FWI3D_ACOUSTIC, par, g);
     float* prev = F;
     float* curr = F + n;
     float* next = F + 2 * n;
@@ -284,8 +276,9 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
     A.dt = dt; A.dt2 = dt * dt;
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
-    const int nblk = tile_grid(A, g, TX);
-    const int nshell = shell_count(6, g.nx, g.ny, g.nz);
+    const int nblk = tile_grid(A, g, TX, 6, g.nz - 6);      // z slabs: acoustic_zslab
+    const int nslab = 12 * g.nx * g.ny;
+    const int nshell = shell_count(4, g.nx, g.ny, g.nz);  // points within 4 cells of a face (SATs)
 
     const int nthr = 256;
     const int nbi = (I.npts * 27 + nthr - 1) / nthr;
@@ -299,30 +292,32 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         const float sdt = (float)sub * dt;
         const int last = nt / sub;
         const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-        const float* fn = snaps + (size_t)itl * n;
-        const float* fm = itl > 0 ? snaps + (size_t)(itl - 1) * n : fn;
-        // a slot past the store reads as zero (the reference dereferences a null slot there)
-        const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * n : nullptr;
         dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
-        hipLaunchKernelGGL(acoustic_grad, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, (const float*)M, g, sdt, branch, gK, gB);
+        with_snap_fmt(c, [&](auto tag) {
+            constexpr int FMT = decltype(tag)::value;
+            // a slot past the store reads as zero (the reference dereferences a null slot there)
+            const SnapView<FMT> fn = snap_view<FMT>(c, itl), fm = itl > 0 ? snap_view<FMT>(c, itl - 1) : fn,
+                                fp = snap_view<FMT>(c, itl + 1);
+            hipLaunchKernelGGL(acoustic_grad<FMT>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, (const float*)M, g, sdt,
+                               branch, gK, gB);
+        });
     };
 
     for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0)
-            HIP_OK(hipMemcpyAsync(snaps + (size_t)(it / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        if (fwd_snap && it % sub == 0) snap_put(c, it / sub, curr);
         if (!adj && E.npts > 0)
             hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it);
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_interior<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_boundary, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_step<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+        timed(c, 3, [&] { hipLaunchKernelGGL(acoustic_zslab, dim3((nslab + 255) / 256), dim3(256), 0, c->stream, A); });
+        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_faces, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
         if (I.npts > 0)
             hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, next, A, I, d_src, nt, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
     }
-    if (fwd_snap && (nt - 1) % sub == 0)
-        HIP_OK(hipMemcpyAsync(snaps + (size_t)((nt - 1) / sub) * n, curr, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c, (nt - 1) / sub, curr);
     if (!adj && E.npts > 0)
         hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1);
     if (adj && d_grad && sub > 0) grad_event(0, curr);
@@ -371,22 +366,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const float dt = par->dt;
     A.dt = dt; A.dt2 = dt * dt;
     const bool fwd_snap = !adj && sub > 0;
-    const long long slot = 3 * n;
-    if (fwd_snap) {
-        const int slots = 1 + nt / sub;
-        c->snaps.ensure((size_t)slots * slot * sizeof(float));
-        HIP_OK(hipMemsetAsync(c->snaps.p, 0, (size_t)slots * slot * sizeof(float), c->stream));
-        c->snap_slots = slots; c->snap_ncomp = 3; c->snap_n = n; c->snap_nt = nt; c->snap_sub = sub;
-        c->snap_physics = vti ? FWI3D_VTI : FWI3D_ELASTIC;
-        c->snap_dims[0] = g.nx; c->snap_dims[1] = g.ny; c->snap_dims[2] = g.nz;
-        c->snap_valid = false;
-    }
-    if (adj && d_grad && sub > 0) {
-        REQUIRE(c->snap_valid && c->snap_physics == (vti ? FWI3D_VTI : FWI3D_ELASTIC) && c->snap_nt == nt &&
-                    c->snap_sub == sub && c->snap_dims[0] == g.nx && c->snap_dims[1] == g.ny && c->snap_dims[2] == g.nz,
-                "adjoint propagation needs the snapshots of a matching forward propagation on this context");
-    }
-    float* snaps = c->snaps.as<float>();
+    const int phys = vti ? FWI3D_VTI : FWI3D_ELASTIC;
+    if (fwd_snap) snap_begin(c, phys, 3, par, g);
+    if (adj && d_grad && sub > 0) snap_check(c, phys, par, g);
     float* prev = F;
     float* curr = F + 3 * n;
     float* next = F + 6 * n;
@@ -403,22 +385,29 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     inject(prev, 0, 1);
 
     constexpr int TX = 4;
-    const int nblk = tile_grid(A, g, TX);
-    const int nshell = shell_count(6, g.nx, g.ny, g.nz);
+    EArgs Af = A;  // flux pass: the whole z range
+    const int nblk_f = tile_grid(Af, g, TX, 0, g.nz);
+    const int nblk = tile_grid(A, g, TX, 6, g.nz - 6);  // z slabs: elastic_zslab
+    const int nslab = 12 * g.nx * g.ny;
+    const int nshell = shell_count(1, g.nx, g.ny, g.nz);  // surface points (traction SATs)
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
         const float sdt = (float)sub * dt;
         const int last = nt / sub;
         const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-        const float* fn = snaps + (size_t)itl * slot;
-        const float* fm = itl > 0 ? snaps + (size_t)(itl - 1) * slot : fn;
-        const float* fp = itl + 1 < c->snap_slots ? snaps + (size_t)(itl + 1) * slot : nullptr;
         dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
-        if (vti)
-            hipLaunchKernelGGL(elastic_grad<true>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
-        else
-            hipLaunchKernelGGL(elastic_grad<false>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt, branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+        with_snap_fmt(c, [&](auto tag) {
+            constexpr int FMT = decltype(tag)::value;
+            const SnapView<FMT> fn = snap_view<FMT>(c, itl), fm = itl > 0 ? snap_view<FMT>(c, itl - 1) : fn,
+                                fp = snap_view<FMT>(c, itl + 1);
+            if (vti)
+                hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt,
+                                   branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+            else
+                hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt,
+                                   branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
+        });
     };
     auto extract = [&](const float* u, int it) {
         if (adj || E.npts == 0) return;
@@ -426,26 +415,28 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
 
     for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0)
-            HIP_OK(hipMemcpyAsync(snaps + (size_t)(it / sub) * slot, curr, slot * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        if (fwd_snap && it % sub == 0) snap_put(c, it / sub, curr);
         extract(curr, it);
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
+        Af.curr = curr; Af.prev = prev; Af.next = next;
+        Af.prev_tapered = A.prev_tapered;
         if (vti) {
-            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<true>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk_f), dim3(64, TX), 0, c->stream, Af); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 3, [&] { hipLaunchKernelGGL(elastic_zslab<true>, dim3((nslab + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_faces<true>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
         } else {
-            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<false>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk_f), dim3(64, TX), 0, c->stream, Af); });
+            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+            timed(c, 3, [&] { hipLaunchKernelGGL(elastic_zslab<false>, dim3((nslab + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_faces<false>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
     }
-    if (fwd_snap && (nt - 1) % sub == 0)
-        HIP_OK(hipMemcpyAsync(snaps + (size_t)((nt - 1) / sub) * slot, curr, slot * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c, (nt - 1) / sub, curr);
     extract(curr, nt - 1);
     if (adj && d_grad && sub > 0) grad_event(0, curr, prev);
     HIP_OK(hipGetLastError());
@@ -508,8 +499,10 @@ int fwi3d_destroy(fwi3d_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
-    c->fields.release(); c->model.release(); c->aux.release(); c->snaps.release();
+    c->fields.release(); c->model.release(); c->aux.release(); c->snaps.release(); c->snapsc.release();
     c->tapers.release(); c->injtab.release(); c->exttab.release(); c->tmpio.release();
+    c->opmodel.release(); c->opscal.release(); c->optrace.release(); c->opgrad.release(); c->opsyn.release();
+    c->opt.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FWI3D_OK;
@@ -646,7 +639,7 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
     }
     c->prof = enable != 0;
     if (enable) {
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 4; k++) {
             c->prof_ms[k] = 0;
             c->prof_n[k] = 0;
         }
@@ -655,7 +648,7 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
 }
 
 int fwi3d_profile_read(const fwi3d_ctx* c, int kernel, double* ms, long long* n) {
-    if (!c || kernel < 0 || kernel > 2) return FWI3D_EINVAL;
+    if (!c || kernel < 0 || kernel > 3) return FWI3D_EINVAL;
     if (ms) *ms = c->prof_ms[kernel];
     if (n) *n = c->prof_n[kernel];
     return FWI3D_OK;

@@ -98,10 +98,121 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
 /* Kernel timing with HIP events on the context's stream, for roofline accounting
  * (no counterpart in the reference). enable=1 starts collecting (and resets), 0 stops.
  * read() returns the summed milliseconds and launch count since the last enable of
- * kernel class `kernel`: 0 = the interior time-step kernel, 1 = the elastic/VTI flux
- * pre-pass, 2 = the boundary-shell time-step kernel. */
+ * kernel class `kernel`: 0 = the time-step kernel (points 6+ cells from the z faces),
+ * 1 = the elastic/VTI flux pre-pass, 2 = the face-SAT kernel, 3 = the z-slab kernel. */
 int fwi3d_profile(fwi3d_ctx *ctx, int enable);
 int fwi3d_profile_read(const fwi3d_ctx *ctx, int kernel, double *ms, long long *launches);
+
+/* ---------------------------------------------------------------------------------
+ * Shot operator: the nonlinear operator of one shot around propagate(), i.e.
+ * nl_we_op_e::apply_forward / apply_jacobianT (src/include/we_op.hpp:50-51, 98-99;
+ * src/cpp/we_op.cpp:872-1078), inherited by nl_we_op_a and nl_we_op_vti, plus the
+ * per-shot part of nlls_fwi::compute_res_and_grad (src/cpp/optimization.cpp:5-197).
+ * ------------------------------------------------------------------------------ */
+
+/* The fields of the reference's `param` (src/include/param.hpp:8-71) that these
+ * operators read or write. In/out fields are updated as analyzeModel does
+ * (src/cpp/we_op.cpp:378-444), so passing the same struct to apply_forward and then
+ * apply_jacobianT reproduces the reference's per-instance `_par`. */
+typedef struct {
+    int nmodels;                  /* 2 (vp, rho), 3 (vp, vs, rho), 6 (vp, vs, rho, delta, epsilon, gamma) */
+    float dt;                     /* in/out: <0 CFL step rounded to divide the data dt, 0 CFL step,
+                                     >0 min(dt, data dt); out: propagation dt */
+    float courant;                /* in/out (default 0.5) */
+    int sub;                      /* in/out: <0 round(data dt / dt), 0 no gradient, >0 imaging interval */
+    int nt;                       /* out: propagation time samples */
+    float vmin, vmax;             /* out: velocity extremes after clipping */
+    int resampling;               /* 0 sinc (default), 1 linear */
+    int sinc_half_length;         /* default 11 */
+    int bc[6];                    /* top,bottom,left,right,front,back (default 1) */
+    int taper[6];                 /* cos^2 taper widths (default 0) */
+    float taper_strength;         /* default 0.05 */
+    float free_surface_stiffness; /* default 1.05 (acoustic: at least 1) */
+    int mt;                       /* 1: moment-tensor source (6 components) */
+    int seismotype;               /* 0 displacement / pressure, 1 its time derivative */
+    int soft_clip;                /* 1: skip the hard clip of elastic/VTI models */
+    float vpmin, vpmax, vsmin, vsmax, rhomin, rhomax;
+    float deltamin, deltamax, epsilonmin, epsilonmax, gammamin, gammamax;
+    int snap_precision;           /* fwi3d_snap_precision */
+} fwi3d_op_param;
+
+/* Geometry of one shot: grid, one source position, receiver positions and the time
+ * axis of the data (the domain/range hypercubes and par.sxyz/rxyz of the reference). */
+typedef struct {
+    int nx, ny, nz;
+    float dx, dy, dz, ox, oy, oz;
+    int nt_data;                  /* samples of source wavelets and data traces */
+    float dt_data;
+    const float *sxyz;            /* [3] source x, y, z (host) */
+    int nrcv;
+    const float *rxyz;            /* [nrcv][3] receiver x, y, z (host) */
+} fwi3d_shot;
+
+/* Fill the defaults of src/include/param.hpp:8-71. */
+void fwi3d_op_param_defaults(fwi3d_op_param *par);
+
+/* analyzeModel (src/cpp/we_op.cpp:234-446) on a host model [nmodels][ny][nx][nz], in
+ * place: hard clip, Thomsen-delta and vp >= sqrt(2) vs fixes, then the time axis
+ * (par->dt, nt, sub, courant, vmin, vmax). */
+int fwi3d_analyze_model(fwi3d_op_param *par, float *model, const fwi3d_shot *shot);
+
+/* apply_forward (src/cpp/we_op.cpp:872-946): raw model [nmodels][ny][nx][nz] and source
+ * wavelets src [nscomp][nt_data] (nscomp 1 acoustic, 3 force, 6 moment tensor) ->
+ * data [nrcomp][nrcv][nt_data] (nrcomp 1 acoustic, 3 elastic), overwritten or added.
+ * Stores the snapshots of the gradient when par->sub > 0 (after analysis). */
+int fwi3d_apply_forward(fwi3d_ctx *ctx, fwi3d_op_param *par, const fwi3d_shot *shot, int add, const float *model,
+                        const float *src, float *data);
+/* apply_jacobianT (src/cpp/we_op.cpp:948-1078): data-space residual -> gradient w.r.t.
+ * the raw (vp, vs, rho) or (vp, rho) parameters, grad [min(3,nmodels)][ny][nx][nz],
+ * overwritten or added. Uses the snapshots of the last apply_forward on ctx. */
+int fwi3d_apply_jacobianT(fwi3d_ctx *ctx, fwi3d_op_param *par, const fwi3d_shot *shot, int add, float *grad,
+                          const float *model0, const float *data);
+
+/* Device-pointer variants; the gradient is an FP64 accumulator. */
+int fwi3d_apply_forward_device(fwi3d_ctx *ctx, fwi3d_op_param *par, const fwi3d_shot *shot, int add,
+                               const float *d_model, const float *d_src, float *d_data);
+int fwi3d_apply_jacobianT_device(fwi3d_ctx *ctx, fwi3d_op_param *par, const fwi3d_shot *shot, int add,
+                                 double *d_grad64, const float *d_model0, const float *d_data);
+
+/* One shot of nlls_fwi::compute_res_and_grad (src/cpp/optimization.cpp:35-188) without
+ * weights, filters or normalisation: forward, residual r = syn - obs (copied to d_resid
+ * when not NULL), the FP64 misfit contribution 0.5 * dt_data * sum_traces (sum_t r^2 -
+ * (r_0^2 + r_{nt-1}^2)/2) (src/include/optimization.hpp:410-424, before the division by
+ * dnorm), then apply_jacobianT(add=1) of Ht r into d_grad64. */
+int fwi3d_fwi_shot_device(fwi3d_ctx *ctx, fwi3d_op_param *par, const fwi3d_shot *shot, const float *d_model,
+                          const float *d_src, const float *d_obs, float *d_resid, double *d_grad64, double *misfit);
+
+/* ---------------------------------------------------------------------------------
+ * Optimiser: l-BFGS with the regular-Wolfe line search (lbfgs::run, updateSY,
+ * computeHg, regular_wolfe::lineSearch/zoom: src/cpp/nlsolver.cpp:10-36, 185-423,
+ * 1158-1359), on device vectors with FP64 dot products and FP64 curvature pairs.
+ * ------------------------------------------------------------------------------ */
+
+/* Objective callback: misfit f and gradient at d_model into d_grad (the caller's own
+ * device buffers, those given to fwi3d_lbfgs_run), like nlls_fwi::res(), which
+ * computes both (src/cpp/optimization.cpp:5-197). Return 0 on success. */
+typedef int (*fwi3d_objective)(void *user, const float *d_model, float *d_grad, double *f);
+
+/* Solver and line-search parameters (src/include/param.hpp:55-62 defaults). */
+typedef struct {
+    int niter;         /* maximum iterations */
+    int m;             /* curvature pairs kept (lbfgs_m, default 5) */
+    int max_trial;     /* line-search trials per iteration (default 10) */
+    double threshold;  /* stop when the relative misfit decrease falls below it (default 0) */
+    double c1, c2;     /* Wolfe constants (1e-4, 0.9) */
+    double a0, a1;     /* first step a0/|g| or a1*|m|/|g| (1, 0) */
+    double step_max;   /* 1e6 */
+    int version;       /* trial-step rule after the first iteration (ls_version, 0) */
+} fwi3d_lbfgs_param;
+
+void fwi3d_lbfgs_param_defaults(fwi3d_lbfgs_param *p);
+
+/* Minimise fn over the FP32 model d_model[n] (updated in place; d_grad holds the
+ * gradient at the returned model). d_h0: optional positive diagonal initial inverse
+ * Hessian (NULL: ones). f_history[niter+1] receives the misfit per accepted iterate,
+ * counters[3] = {iterations, function evaluations, gradient evaluations}. */
+int fwi3d_lbfgs_run(fwi3d_ctx *ctx, const fwi3d_lbfgs_param *par, long long n, float *d_model, float *d_grad,
+                    const float *d_h0, fwi3d_objective fn, void *user, double *f_history, int *counters);
 
 #ifdef __cplusplus
 }

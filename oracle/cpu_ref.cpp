@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only (see sbp.hpp). C-ABI of the CPU restatement,
 // built by oracle/Makefile into oracle/_build/libcpu_ref.so. Every function has a
 // float (_f) and a double (_d) variant; the double build is the FP64 oracle.
-#include "wave.hpp"
+#include "pipeline.hpp"
 
 extern "C" {
 
@@ -19,6 +19,30 @@ typedef struct {
     int bc[6], taper[6];
     double taper_strength, free_surface_stiffness;
 } orc_par;
+
+// Shot-operator parameters (src/include/param.hpp:8-71): in/out like the reference's _par.
+typedef struct {
+    int nmodels;
+    double dt, courant;
+    int sub, nt;
+    double vmin, vmax;
+    int resampling, sinc_half_length;
+    int bc[6], taper[6];
+    double taper_strength, free_surface_stiffness;
+    int mt, seismotype, soft_clip;
+    double vpmin, vpmax, vsmin, vsmax, rhomin, rhomax, deltamin, deltamax, epsilonmin, epsilonmax, gammamin, gammamax;
+} orc_op_par;
+
+// Geometry of one shot; positions are arrays of the variant's type.
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, ox, oy, oz;
+    int nt_data;
+    double dt_data;
+    const void* sxyz;
+    int nrcv;
+    const void* rxyz;
+} orc_shot;
 }
 
 namespace {
@@ -71,7 +95,99 @@ int propagate_t(int physics, int adj, const T* model, const T* src, T* rcv, cons
     }
 }
 
+template <class T>
+orc::OpParams<T> to_op(const orc_op_par* p) {
+    orc::OpParams<T> q;
+    q.nmodels = p->nmodels; q.dt = (T)p->dt; q.courant = (T)p->courant; q.sub = p->sub; q.nt = p->nt;
+    q.vmin = (T)p->vmin; q.vmax = (T)p->vmax; q.resampling = p->resampling; q.sinc_half_length = p->sinc_half_length;
+    for (int i = 0; i < 6; i++) { q.bc[i] = p->bc[i]; q.taper[i] = p->taper[i]; }
+    q.taper_strength = (T)p->taper_strength; q.free_surface_stiffness = (T)p->free_surface_stiffness;
+    q.mt = p->mt; q.seismotype = p->seismotype; q.soft_clip = p->soft_clip;
+    q.vpmin = (T)p->vpmin; q.vpmax = (T)p->vpmax; q.vsmin = (T)p->vsmin; q.vsmax = (T)p->vsmax;
+    q.rhomin = (T)p->rhomin; q.rhomax = (T)p->rhomax; q.deltamin = (T)p->deltamin; q.deltamax = (T)p->deltamax;
+    q.epsilonmin = (T)p->epsilonmin; q.epsilonmax = (T)p->epsilonmax; q.gammamin = (T)p->gammamin; q.gammamax = (T)p->gammamax;
+    return q;
+}
+
+template <class T>
+void from_op(const orc::OpParams<T>& q, orc_op_par* p) {
+    p->nmodels = q.nmodels; p->dt = q.dt; p->courant = q.courant; p->sub = q.sub; p->nt = q.nt;
+    p->vmin = q.vmin; p->vmax = q.vmax; p->mt = q.mt; p->soft_clip = q.soft_clip;
+    p->free_surface_stiffness = q.free_surface_stiffness;
+    for (int i = 0; i < 6; i++) p->bc[i] = q.bc[i];
+}
+
+template <class T>
+orc::ShotGeom<T> to_shot(const orc_shot* s) {
+    orc::ShotGeom<T> G;
+    G.g = orc::Grid{s->nx, s->ny, s->nz};
+    G.dx = (T)s->dx; G.dy = (T)s->dy; G.dz = (T)s->dz; G.ox = (T)s->ox; G.oy = (T)s->oy; G.oz = (T)s->oz;
+    G.nt_data = s->nt_data; G.dt_data = (T)s->dt_data;
+    G.sxyz = (const T*)s->sxyz; G.nrcv = s->nrcv; G.rxyz = (const T*)s->rxyz;
+    return G;
+}
+
 }  // namespace
+
+#define ORC_OP_EXPORTS(T, S)                                                                                           \
+    int orc_analyze_##S(orc_op_par* par, T* model, const orc_shot* shot) {                                             \
+        auto P = to_op<T>(par);                                                                                        \
+        auto G = to_shot<T>(shot);                                                                                     \
+        orc::normalize_params(P);                                                                                      \
+        orc::analyze_model<T>(model, G.g, G.dx, G.dy, G.dz, G.nt_data, G.dt_data, P);                                  \
+        from_op<T>(P, par);                                                                                            \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_convert_##S(T* model, long long n, int nmodels) {                                                          \
+        orc::convert_model<T>(model, (size_t)n, nmodels);                                                              \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_resample_##S(int method, int adj, int add, T* mod, T* dat, int ntr, int nm, T dm, int nd, T dd, int hl) {  \
+        if (method == 1) orc::linear_resample<T>(adj, add, mod, dat, ntr, nm, dm, nd, dd);                            \
+        else orc::sinc_resample<T>(adj, add, mod, dat, ntr, nm, dm, nd, dd, hl);                                       \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_ht_##S(int inv, T* x, int ntr, int nt, T dt) {                                                             \
+        orc::apply_ht<T>(inv, x, ntr, nt, dt);                                                                         \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_dt_##S(int adj, const T* in, T* out, int ntr, int nt, T dt) {                                              \
+        orc::dt_op<T>(adj, in, out, ntr, nt, dt);                                                                      \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_apply_forward_##S(orc_op_par* par, const orc_shot* shot, int add, const T* model, const T* src, T* data,   \
+                              T* snaps) {                                                                              \
+        try {                                                                                                          \
+            auto P = to_op<T>(par);                                                                                    \
+            orc::apply_forward<T>(P, to_shot<T>(shot), model, src, add, data, snaps);                                  \
+            from_op<T>(P, par);                                                                                        \
+            return 0;                                                                                                  \
+        } catch (...) {                                                                                                \
+            return -1;                                                                                                 \
+        }                                                                                                              \
+    }                                                                                                                  \
+    int orc_apply_jacobianT_##S(orc_op_par* par, const orc_shot* shot, int add, T* grad, const T* model0,             \
+                                const T* data, const T* snaps) {                                                       \
+        try {                                                                                                          \
+            auto P = to_op<T>(par);                                                                                    \
+            orc::apply_jacobianT<T>(P, to_shot<T>(shot), model0, data, add, grad, snaps);                              \
+            from_op<T>(P, par);                                                                                        \
+            return 0;                                                                                                  \
+        } catch (...) {                                                                                                \
+            return -1;                                                                                                 \
+        }                                                                                                              \
+    }                                                                                                                  \
+    int orc_fwi_shot_##S(orc_op_par* par, const orc_shot* shot, const T* model, const T* src, const T* obs, T* resid,  \
+                         T* grad, T* snaps, double* misfit) {                                                          \
+        try {                                                                                                          \
+            auto P = to_op<T>(par);                                                                                    \
+            *misfit = orc::fwi_shot<T>(P, to_shot<T>(shot), model, src, obs, resid, grad, snaps);                      \
+            from_op<T>(P, par);                                                                                        \
+            return 0;                                                                                                  \
+        } catch (...) {                                                                                                \
+            return -1;                                                                                                 \
+        }                                                                                                              \
+    }
 
 #define ORC_EXPORTS(T, S)                                                                                              \
     int orc_d1_##S(int axis, int add, const T* in, T* out, int nx, int ny, int nz, T h, const T* par) {               \
@@ -153,4 +269,6 @@ int propagate_t(int physics, int adj, const T* model, const T* src, T* rcv, cons
 extern "C" {
 ORC_EXPORTS(float, f)
 ORC_EXPORTS(double, d)
+ORC_OP_EXPORTS(float, f)
+ORC_OP_EXPORTS(double, d)
 }

@@ -204,3 +204,163 @@ def propagate(physics, model, src, inj, ext, par, d, adj=False, grad=None, snaps
     if rc != 0:
         raise RuntimeError("oracle propagate failed")
     return rcv, snaps
+
+
+# ---------------------------------------------------------------------------------
+# Shot operator (apply_forward / apply_jacobianT / FWI shot) of the restatement.
+
+class OpPar(C.Structure):
+    """orc_op_par: the reference's param fields read by the shot operator (double)."""
+    _fields_ = [("nmodels", C.c_int), ("dt", C.c_double), ("courant", C.c_double), ("sub", C.c_int),
+                ("nt", C.c_int), ("vmin", C.c_double), ("vmax", C.c_double), ("resampling", C.c_int),
+                ("sinc_half_length", C.c_int), ("bc", C.c_int * 6), ("taper", C.c_int * 6),
+                ("taper_strength", C.c_double), ("free_surface_stiffness", C.c_double), ("mt", C.c_int),
+                ("seismotype", C.c_int), ("soft_clip", C.c_int)] + \
+               [(k, C.c_double) for k in ("vpmin", "vpmax", "vsmin", "vsmax", "rhomin", "rhomax", "deltamin",
+                                          "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")]
+
+
+class ShotC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+               [(k, C.c_double) for k in ("dx", "dy", "dz", "ox", "oy", "oz")] + \
+               [("nt_data", C.c_int), ("dt_data", C.c_double), ("sxyz", C.c_void_p), ("nrcv", C.c_int),
+                ("rxyz", C.c_void_p)]
+
+
+# defaults of the reference's param (src/include/param.hpp:8-71)
+OP_DEFAULTS = dict(nmodels=3, dt=-1.0, courant=0.5, sub=0, nt=0, vmin=0.1, vmax=8.0, resampling=0,
+                   sinc_half_length=11, bc=(1,) * 6, taper=(0,) * 6, taper_strength=0.05,
+                   free_surface_stiffness=1.05, mt=0, seismotype=0, soft_clip=0, vpmin=0.2, vpmax=8.0, vsmin=0.1,
+                   vsmax=5.0, rhomin=0.1, rhomax=8.0, deltamin=-0.5, deltamax=1.0, epsilonmin=0.0, epsilonmax=1.0,
+                   gammamin=0.0, gammamax=1.0)
+
+
+def op_par(**kw):
+    v = dict(OP_DEFAULTS)
+    v.update(kw)
+    p = OpPar()
+    for k, x in v.items():
+        if k in ("bc", "taper"):
+            setattr(p, k, (C.c_int * 6)(*x))
+        else:
+            setattr(p, k, x)
+    return p
+
+
+class ShotGeom:
+    """Geometry of one shot with its position arrays kept alive in the variant's type."""
+
+    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz, s="f"):
+        T = _dt(s)
+        self.s = s
+        self.shape = tuple(shape)
+        self.sxyz = np.ascontiguousarray(np.asarray(sxyz, T).reshape(3))
+        self.rxyz = np.ascontiguousarray(np.asarray(rxyz, T).reshape(-1, 3))
+        ny, nx, nz = shape
+        self.c = ShotC(nx, ny, nz, d[0], d[1], d[2], o[0], o[1], o[2], int(nt_data), float(dt_data),
+                       self.sxyz.ctypes.data, self.rxyz.shape[0], self.rxyz.ctypes.data)
+        self.nrcv = self.rxyz.shape[0]
+        self.nt_data = int(nt_data)
+
+
+def _ncomp(nmodels):
+    return 1 if nmodels == 2 else 3
+
+
+def analyze(par, model, geom):
+    """analyzeModel on a copy of model; returns (clipped model, par updated in place)."""
+    T = _dt(geom.s)
+    m = np.ascontiguousarray(model, dtype=T).copy()
+    getattr(lib(), f"orc_analyze_{geom.s}")(C.byref(par), _ptr(m), C.byref(geom.c))
+    return m
+
+
+def convert(model, nmodels, s="f"):
+    m = np.ascontiguousarray(model, dtype=_dt(s)).copy()
+    n = m.size // nmodels
+    getattr(lib(), f"orc_convert_{s}")(_ptr(m), C.c_longlong(n), nmodels)
+    return m
+
+
+def resample(method, adj, add, mod, dat, dm, dd, hl=11, s="f"):
+    """Resampler between model traces mod [ntr][nm] (step dm) and data traces dat
+    [ntr][nd] (step dd); fwd writes dat, adj writes mod (in place)."""
+    cT = C.c_float if s == "f" else C.c_double
+    ntr, nm = mod.shape
+    nd = dat.shape[1]
+    getattr(lib(), f"orc_resample_{s}")(method, int(adj), int(add), _ptr(mod), _ptr(dat), ntr, nm, cT(dm), nd, cT(dd), hl)
+
+
+def ht(inv, x, dt, s="f"):
+    cT = C.c_float if s == "f" else C.c_double
+    ntr, nt = x.shape
+    getattr(lib(), f"orc_ht_{s}")(int(inv), _ptr(x), ntr, nt, cT(dt))
+    return x
+
+
+def time_derivative(adj, x, dt, s="f"):
+    cT = C.c_float if s == "f" else C.c_double
+    out = np.zeros_like(x)
+    ntr, nt = x.shape
+    getattr(lib(), f"orc_dt_{s}")(int(adj), _ptr(x), _ptr(out), ntr, nt, cT(dt))
+    return out
+
+
+def _snap_buffer(par, model, geom):
+    """Snapshot store sized from an analysis on a copy of par."""
+    p2 = OpPar.from_buffer_copy(par)
+    analyze(p2, model, geom)
+    ny, nx, nz = geom.shape
+    sub = p2.sub
+    if sub <= 0:
+        return None
+    return np.zeros((n_snap_slots(p2.nt, sub), _ncomp(par.nmodels), ny, nx, nz), _dt(geom.s))
+
+
+def apply_forward(par, geom, model, src, add=False, data=None):
+    """apply_forward: raw model, wavelets src [nscomp][nt_data] -> data
+    [nrcomp][nrcv][nt_data]. Returns (data, snaps); par is updated in place."""
+    T = _dt(geom.s)
+    model = np.ascontiguousarray(model, dtype=T)
+    src = np.ascontiguousarray(src, dtype=T)
+    if data is None:
+        data = np.zeros((_ncomp(par.nmodels), geom.nrcv, geom.nt_data), T)
+    snaps = _snap_buffer(par, model, geom)
+    rc = getattr(lib(), f"orc_apply_forward_{geom.s}")(C.byref(par), C.byref(geom.c), int(add), _ptr(model), _ptr(src),
+                                                      _ptr(data), _ptr(snaps))
+    if rc != 0:
+        raise RuntimeError("oracle apply_forward failed")
+    return data, snaps
+
+
+def apply_jacobianT(par, geom, model0, data, snaps, add=False, grad=None):
+    T = _dt(geom.s)
+    ny, nx, nz = geom.shape
+    model0 = np.ascontiguousarray(model0, dtype=T)
+    data = np.ascontiguousarray(data, dtype=T)
+    if grad is None:
+        grad = np.zeros((min(3, par.nmodels), ny, nx, nz), T)
+    rc = getattr(lib(), f"orc_apply_jacobianT_{geom.s}")(C.byref(par), C.byref(geom.c), int(add), _ptr(grad),
+                                                        _ptr(model0), _ptr(data), _ptr(snaps))
+    if rc != 0:
+        raise RuntimeError("oracle apply_jacobianT failed")
+    return grad
+
+
+def fwi_shot(par, geom, model, src, obs, grad=None):
+    """Returns (misfit, residual, grad); grad accumulates."""
+    T = _dt(geom.s)
+    ny, nx, nz = geom.shape
+    model = np.ascontiguousarray(model, dtype=T)
+    src = np.ascontiguousarray(src, dtype=T)
+    obs = np.ascontiguousarray(obs, dtype=T)
+    resid = np.zeros_like(obs)
+    if grad is None:
+        grad = np.zeros((min(3, par.nmodels), ny, nx, nz), T)
+    snaps = _snap_buffer(par, model, geom)
+    mis = C.c_double()
+    rc = getattr(lib(), f"orc_fwi_shot_{geom.s}")(C.byref(par), C.byref(geom.c), _ptr(model), _ptr(src), _ptr(obs),
+                                                 _ptr(resid), _ptr(grad), _ptr(snaps), C.byref(mis))
+    if rc != 0:
+        raise RuntimeError("oracle fwi_shot failed")
+    return mis.value, resid, grad

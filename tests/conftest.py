@@ -27,3 +27,15 @@ def pytest_configure(config):
 def rng():
     import numpy as np
     return np.random.default_rng(1234)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _one_hip_runtime(request):
+    """PyTorch-ROCm bundles its own libamdhip64.so.7 (same soname as the system one the
+    library links). Initialising torch before the library is first loaded makes both
+    bind to that one runtime; the other order leaves torch without a device."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield

@@ -78,3 +78,34 @@ def test_injector_moment_conditions():
         we = inj.xw[0, 1].astype(np.float64)
         for k in range(3):
             assert abs(np.sum(we * xi ** k) - x ** k) < 1e-6
+
+
+@pytest.mark.parametrize("nmodels", [2, 3, 6])
+@pytest.mark.parametrize("dt,sub", [(-1.0, -1), (0.0, 0), (0.0015, -1), (0.01, 3)])
+def test_host_analyze_model_matches_oracle(nmodels, dt, sub):
+    """fwi3d_analyze_model (host code of the library) against the oracle's analyzeModel:
+    the same clipped model bit for bit and the same time axis."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(nmodels * 10 + sub)
+    shape = (13, 14, 15)
+    vp = rng.uniform(0.1, 9.0, shape).astype(np.float32)      # some outside [0.2, 8]
+    vs = (vp / rng.uniform(1.2, 2.2, shape)).astype(np.float32)  # some above vp/sqrt(2)
+    rho = rng.uniform(0.05, 3.0, shape).astype(np.float32)
+    if nmodels == 2:
+        m = np.stack([vp, rho])
+    elif nmodels == 3:
+        m = np.stack([vp, vs, rho])
+    else:
+        m = np.stack([vp, vs, rho, rng.uniform(-0.6, 1.1, shape), rng.uniform(-0.1, 1.1, shape),
+                      rng.uniform(-0.1, 1.1, shape)]).astype(np.float32)
+    kw = dict(nmodels=nmodels, dt=dt, sub=sub, courant=0.45)
+    p = F.op_param(**kw)
+    shot = F.Shot(shape, (0.01, 0.012, 0.011), (0, 0, 0), 301, 0.002, (0.05, 0.05, 0.05), [(0.06, 0.06, 0.02)])
+    got = F.analyze_model(p, m, shot)
+    q = O.op_par(**kw)
+    g = O.ShotGeom(shape, (0.01, 0.012, 0.011), (0, 0, 0), 301, 0.002, (0.05, 0.05, 0.05), [(0.06, 0.06, 0.02)], s="f")
+    want = O.analyze(q, m, g)
+    assert np.array_equal(got, want)
+    assert (p.nt, p.sub) == (q.nt, q.sub)
+    assert np.float32(p.dt) == np.float32(q.dt) and np.float32(p.courant) == np.float32(q.courant)
+    assert np.float32(p.vmax) == np.float32(q.vmax) and np.float32(p.vmin) == np.float32(q.vmin)
