@@ -243,7 +243,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = {name: eng.profile_read(k)
-            for k, name in ((0, "elastic_step"), (1, "elastic_flux"), (2, "elastic_faces"), (3, "elastic_zslab"))}
+            for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"))}
     eng.profile(False)
 
     g = grad.cpu().numpy()
@@ -251,7 +251,7 @@ def main():
 
     updates = 2.0 * N * (nt - 1) * args.steps  # forward + adjoint cell-updates
     gcell = updates / elapsed / 1e9
-    # one elastic time step = flux pre-pass + y-march step + z slabs + face SATs launches
+    # one elastic time step = flux pre-pass + interior y-march + boundary shell launches
     avg_us = {k: ms / max(n, 1) * 1e3 for k, (ms, n) in prof.items()}
     t_pair = sum(avg_us.values()) / 1e6
     achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
@@ -286,7 +286,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": f"profiles/elastic_{args.n}cube_step_traffic_pmc.md" if traffic else None,
-            "kernel": "elastic time step = elastic_flux + elastic_step + elastic_zslab + elastic_faces launches "
+            "kernel": "elastic time step = elastic_flux + elastic_interior + elastic_boundary launches "
                       "(HIP events on the engine stream)",
             "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
             "cells_per_step": N,

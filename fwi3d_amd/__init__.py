@@ -26,7 +26,7 @@ EXPORTS = (
     "fwi3d_propagate", "fwi3d_propagate_device", "fwi3d_profile", "fwi3d_profile_read",
     "fwi3d_op_param_defaults", "fwi3d_analyze_model", "fwi3d_apply_forward", "fwi3d_apply_jacobianT",
     "fwi3d_apply_forward_device", "fwi3d_apply_jacobianT_device", "fwi3d_fwi_shot_device",
-    "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run",
+    "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run", "fwi3d_tune",
 )
 
 
@@ -213,6 +213,13 @@ class Engine:
                                           C.c_void_p(grad64_ptr) if grad64_ptr else None, C.byref(par), nx, ny, nz,
                                           C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device")
+
+    def tune(self, ychunk=None, tx=None):
+        """Tiling of the time-step kernels (fwi3d_tune); results do not depend on it."""
+        if ychunk is not None:
+            self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")
+        if tx is not None:
+            self._check(lib().fwi3d_tune(self._h, 1, int(tx)), "fwi3d_tune")
 
     def profile(self, enable=True):
         self._check(lib().fwi3d_profile(self._h, int(bool(enable))), "fwi3d_profile")

@@ -1,13 +1,12 @@
 // Acoustic variable-density kernels (gfx950).
 //
-// One time step of nl_we_op_a::propagate (src/cpp/we_op.cpp:1662-1791) is three
-// launches that only read curr/prev/model:
-//   acoustic_step   the points 6+ cells from the z faces: D2(1/rho) along x, y, z
-//                   (interior stencil; SBP closure rows for x rows / y planes near a
-//                   face) from an LDS x-z plane and a register queue along y, the
-//                   leapfrog update, the absorbing scaling and the taper;
-//   acoustic_zslab  the same for the 6-deep z slabs, one thread per point;
-//   acoustic_faces  adds the face SAT terms on the points within 4 cells of a face.
+// One time step of nl_we_op_a::propagate (src/cpp/we_op.cpp:1662-1791) is two
+// launches that write disjoint points of `next` and only read curr/prev/model:
+//   acoustic_interior  points 6+ cells inside every face: D2(1/rho) along x, y, z
+//                      from an LDS x-z plane and a register queue along y, the
+//                      leapfrog update and the cos^2 taper;
+//   acoustic_boundary  the 6-cell shell: SBP closure rows, the face SATs, the update,
+//                      the absorbing-boundary scaling and the taper.
 // Source injection is a separate tiny launch that adds its already-updated
 // contribution (the update is linear in the injected term); receiver extraction
 // reads curr, which the step does not modify.
@@ -36,43 +35,10 @@ struct AcuArgs {
     float dt, dt2, alpha;  // alpha = 0.2508560249 / free_surface_stiffness
     int prev_tapered;
     int ychunk, ntz, ntx, nyc;
-    int zlo, zhi;  // z range of the y-marching kernel (z tiles start at zlo)
 };
 
-// Divisor of the absorbing SAT's implicit part at a boundary point (1 elsewhere).
-__device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float Kp, float bp, int ix, int iy, int iz, float dt) {
-    float sc = 0.f;
-    bool any = false;
-    const float c = sqrtf(Kp * bp);
-    if (iz == 0 && bc[0] == 2) { sc += c * dt / (2.f * g.dz * SAT_H0); any = true; }
-    if (iz == g.nz - 1 && bc[1] == 2) { sc += c * dt / (2.f * g.dz * SAT_H0); any = true; }
-    if (ix == 0 && bc[2] == 2) { sc += c * dt / (2.f * g.dx * SAT_H0); any = true; }
-    if (ix == g.nx - 1 && bc[3] == 2) { sc += c * dt / (2.f * g.dx * SAT_H0); any = true; }
-    if (iy == 0 && bc[4] == 2) { sc += c * dt / (2.f * g.dy * SAT_H0); any = true; }
-    if (iy == g.ny - 1 && bc[5] == 2) { sc += c * dt / (2.f * g.dy * SAT_H0); any = true; }
-    return any ? 1.f + sc : 1.f;
-}
-
-// Closure-row bookkeeping of one axis for a point at index i of n: the D2 row
-// (-1 inside), the D1 row (-1 inside), and the side.
-struct Rows {
-    int r2, r1;
-    bool lo;
-    __device__ Rows(int i, int n) {
-        lo = i < 6;
-        r2 = i < 6 ? i : (i >= n - 6 ? n - 1 - i : -1);
-        r1 = i < 4 ? i : (i >= n - 4 ? n - 1 - i : -1);
-    }
-};
-
-// One time step on the points 6+ cells from the z faces (z in [6, nz-6)), every x and
-// y: D2(1/rho) along x, y, z from an LDS x-z plane and a y register queue (SBP
-// closure rows of the x rows and y planes near a face from L2-resident global memory:
-// wave- and iteration-uniform branches), the leapfrog update, the absorbing scaling
-// and the cos^2 taper. The z slabs are acoustic_zslab's; the face SATs follow in
-// acoustic_faces (the update is linear in them).
 template <int TX>
-__global__ __launch_bounds__(64 * TX) void acoustic_step(AcuArgs A) {
+__global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
     constexpr int TZ = 64;
     __shared__ float su[TX + 4][TZ + 4];
     __shared__ float sb[TX + 4][TZ + 4];
@@ -93,22 +59,24 @@ __global__ __launch_bounds__(64 * TX) void acoustic_step(AcuArgs A) {
     const int tz = L % A.ntz;
     const int tx = (L / A.ntz) % A.ntx;
     const int cy = L / (A.ntz * A.ntx);
-    // blockDim.x == 64: each wave is one x row, so lx is wave-uniform
-    const int lz = threadIdx.x, lx = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int z0 = A.zlo + tz * TZ, x0 = tx * TX;
+    const int lz = threadIdx.x, lx = threadIdx.y;
+    const int z0 = tz * TZ, x0 = tx * TX;
     const int iz = z0 + lz, ix = x0 + lx;
     const int y0 = cy * A.ychunk;
     const int y1 = min(ny, y0 + A.ychunk);
+    // every load uses clamped coordinates: values outside the grid are never used
+    // by an interior point, so no load needs a guard
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
-    const bool valid = ix < nx && iz < A.zhi;
-    const Rows RX(ix, nx);
+    const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
     auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
     HaloMap<TX> H;
     H.init(lx, lz, x0, z0, nx, nz, sx);
 
+    // y queues y-2..y+2 plus the plane y+3, loaded one iteration ahead of its use
     float um2 = ldy(curr, y0 - 2), um1 = ldy(curr, y0 - 1), u0 = ldy(curr, y0), up1 = ldy(curr, y0 + 1), up2 = ldy(curr, y0 + 2), up3;
     float bm2 = ldy(bb, y0 - 2), bm1 = ldy(bb, y0 - 1), b0 = ldy(bb, y0), bp1 = ldy(bb, y0 + 1), bp2 = ldy(bb, y0 + 2), bp3;
+    // halo values of the plane to stage next, loaded one iteration ahead
     float hxu = 0.f, hxb = 0.f, hzu = 0.f, hzb = 0.f;
     auto prefetch = [&](int y) {
         const int pl = min(y, ny - 1) * sy;
@@ -128,23 +96,11 @@ __global__ __launch_bounds__(64 * TX) void acoustic_step(AcuArgs A) {
         if (H.has_z) { su[H.zrow][H.zcol] = hzu; sb[H.zrow][H.zcol] = hzb; }
         prefetch(iy + 1);
         __syncthreads();
-        if (valid) {
+        if (comp_xz && iy >= 6 && iy < ny - 6) {
             const int p = pl + col;
-            const Rows RY(iy, ny);
-            float sx_, sy_;
-            if (RX.r2 < 0) {
-                sx_ = d2_interior(su[lx][lz + 2], su[lx + 1][lz + 2], u0, su[lx + 3][lz + 2], su[lx + 4][lz + 2],
-                                  sb[lx][lz + 2], sb[lx + 1][lz + 2], b0, sb[lx + 3][lz + 2], sb[lx + 4][lz + 2]);
-            } else {
-                const int base = pl + (RX.lo ? 0 : (nx - 1) * sx) + cz, st = RX.lo ? sx : -sx;
-                sx_ = d2_rows(RX.r2, [&](int j) { return curr[base + j * st]; }, [&](int k) { return bb[base + k * st]; });
-            }
-            if (RY.r2 < 0) {
-                sy_ = d2_interior(um2, um1, u0, up1, up2, bm2, bm1, b0, bp1, bp2);
-            } else {
-                const int base = (RY.lo ? 0 : (ny - 1) * sy) + col, st = RY.lo ? sy : -sy;
-                sy_ = d2_rows(RY.r2, [&](int j) { return curr[base + j * st]; }, [&](int k) { return bb[base + k * st]; });
-            }
+            const float sx_ = d2_interior(su[lx][lz + 2], su[lx + 1][lz + 2], u0, su[lx + 3][lz + 2], su[lx + 4][lz + 2],
+                                          sb[lx][lz + 2], sb[lx + 1][lz + 2], b0, sb[lx + 3][lz + 2], sb[lx + 4][lz + 2]);
+            const float sy_ = d2_interior(um2, um1, u0, up1, up2, bm2, bm1, b0, bp1, bp2);
             const float sz_ = d2_interior(su[lx + 2][lz], su[lx + 2][lz + 1], u0, su[lx + 2][lz + 3], su[lx + 2][lz + 4],
                                           sb[lx + 2][lz], sb[lx + 2][lz + 1], b0, sb[lx + 2][lz + 3], sb[lx + 2][lz + 4]);
             float lap = idx2 * sx_;
@@ -152,9 +108,7 @@ __global__ __launch_bounds__(64 * TX) void acoustic_step(AcuArgs A) {
             lap = lap + idz2 * sz_;
             float pv = prev[p];
             if (ptap) pv = taper_apply(pv, tz2, tx2, ty2, ix, iy, iz);
-            const float k = K[p];
-            float nv = dt2 * lap * k + 2.f * u0 - pv;
-            if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1) nv = nv / acu_scale(A.g, A.bc, k, b0, ix, iy, iz, A.dt);
+            const float nv = dt2 * lap * K[p] + 2.f * u0 - pv;
             next[p] = taper_apply(nv, tz2, tx2, ty2, ix, iy, iz);
         }
         um2 = um1; um1 = u0; u0 = up1; up1 = up2; up2 = up3;
@@ -162,41 +116,35 @@ __global__ __launch_bounds__(64 * TX) void acoustic_step(AcuArgs A) {
     }
 }
 
-// The z slabs (z < 6 or z >= nz-6): one thread per point, operands from global memory
-// (closure rows along z, and along x / y near those faces), update, scaling, taper.
-__global__ __launch_bounds__(256) void acoustic_zslab(AcuArgs A) {
+// Divisor of the absorbing SAT's implicit part at a boundary point (1 elsewhere).
+__device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float Kp, float bp, int ix, int iy, int iz, float dt) {
+    float sc = 0.f;
+    bool any = false;
+    const float c = sqrtf(Kp * bp);
+    if (iz == 0 && bc[0] == 2) { sc += c * dt / (2.f * g.dz * SAT_H0); any = true; }
+    if (iz == g.nz - 1 && bc[1] == 2) { sc += c * dt / (2.f * g.dz * SAT_H0); any = true; }
+    if (ix == 0 && bc[2] == 2) { sc += c * dt / (2.f * g.dx * SAT_H0); any = true; }
+    if (ix == g.nx - 1 && bc[3] == 2) { sc += c * dt / (2.f * g.dx * SAT_H0); any = true; }
+    if (iy == 0 && bc[4] == 2) { sc += c * dt / (2.f * g.dy * SAT_H0); any = true; }
+    if (iy == g.ny - 1 && bc[5] == 2) { sc += c * dt / (2.f * g.dy * SAT_H0); any = true; }
+    return any ? 1.f + sc : 1.f;
+}
+
+// One boundary-shell point: closures, SATs (faces in the reference's order top,
+// bottom, left, right, front, back), update, scaling, taper.
+__global__ __launch_bounds__(256) void acoustic_boundary(AcuArgs A) {
     const Geom g = A.g;
     int ix, iy, iz;
-    if (!zslab_point(blockIdx.x * blockDim.x + threadIdx.x, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
     const float* __restrict__ curr = A.curr;
     const float* __restrict__ bb = A.b;
+    const float* __restrict__ K = A.K;
     const int p = g.at(ix, iy, iz);
     float lap = (1.0f / (g.dx * g.dx)) * d2_point(curr, bb, p, ix, g.nx, g.sx);
     lap = lap + (1.0f / (g.dy * g.dy)) * d2_point(curr, bb, p, iy, g.ny, g.sy);
     lap = lap + (1.0f / (g.dz * g.dz)) * d2_point(curr, bb, p, iz, g.nz, 1);
     float pv = A.prev[p];
     if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-    const float k = A.K[p];
-    float nv = A.dt2 * lap * k + 2.f * curr[p] - pv;
-    nv = nv / acu_scale(g, A.bc, k, bb[p], ix, iy, iz, A.dt);
-    A.next[p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-}
-
-// Face SATs (faces in the reference's order top, bottom, left, right, front, back)
-// on the points within 4 cells of a face, added to the updated field:
-// next += taper(dt^2 K sat / scale). Weak free surface (bc 1) reaches 4 points deep;
-// absorbing (2) and rigid (3) act on the face points.
-__global__ __launch_bounds__(256) void acoustic_faces(AcuArgs A) {
-    const Geom g = A.g;
-    int ix, iy, iz;
-    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 4, g.nx, g.ny, g.nz, ix, iy, iz)) return;
-    const float* __restrict__ curr = A.curr;
-    const float* __restrict__ bb = A.b;
-    const float* __restrict__ K = A.K;
-    const int p = g.at(ix, iy, iz);
-    float pv = A.prev[p];
-    if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-    bool any = false;
     float sat = 0.f;
 #pragma unroll
     for (int f = 0; f < 6; f++) {
@@ -217,19 +165,17 @@ __global__ __launch_bounds__(256) void acoustic_faces(AcuArgs A) {
             const float d2 = h * h;
             sat += 1.0f / (c_hnorm[k] * d2) * c_sat_s[k] * v;
             if (k == 0) sat -= v / (A.alpha * SAT_H0 * d2);
-            any = true;
         } else if (k == 0) {
             const float sd = sat_s(curr, p, inward);
             const float imp = (bc == 2) ? 0.5f / sqrtf(K[p] / bb[p]) : 0.f;
             const float adh = 1.0f / (SAT_H0 * h);
             sat -= adh * (bb[p] * sd / h - imp * pv / A.dt);
-            any = true;
         }
     }
-    if (!any) return;
-    float add = A.dt2 * sat * K[p];
-    add = add / acu_scale(g, A.bc, K[p], bb[p], ix, iy, iz, A.dt);
-    A.next[p] += taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+    lap += sat;
+    float nv = A.dt2 * lap * K[p] + 2.f * curr[p] - pv;
+    nv = nv / acu_scale(g, A.bc, K[p], bb[p], ix, iy, iz, A.dt);
+    A.next[p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
 }
 
 // ------------------------------------------------------------------------------------

@@ -58,61 +58,6 @@ __device__ __forceinline__ float d2_interior(float um2, float um1, float u0, flo
            (D2C7 * bm2 + D2C8 * bm1 + D2C9 * b0) * um2 + (D2C7 * bp2 + D2C8 * bp1 + D2C9 * b0) * up2;
 }
 
-// D2(b) (times h^2) on a boundary row (i < 6 or i >= n-6), reading global memory.
-// Used by the boundary-shell kernels only.
-__device__ __forceinline__ float d2_closure(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
-                                            int s) {
-    const bool lo = i < 6;
-    const int r = lo ? i : n - 1 - i;
-    // position of closure point k: lo -> k, hi -> n-1-k
-    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
-    const int step = lo ? s : -s;
-    float bk[8], uk[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        bk[k] = b[base + k * step];
-        uk[k] = u[base + k * step];
-    }
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        float w = 0.f;
-#pragma unroll
-        for (int k = 0; k < 8; k++) w += c_d2b[r][j][k] * bk[k];
-        acc += w * uk[j];
-    }
-    return acc;
-}
-
-// D2(b) (times h^2) at axis index i (any row) reading global memory: the field and
-// coefficient at point p, axis length n, element stride s.
-__device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
-                                          int s) {
-    if (i >= 6 && i < n - 6) {
-        return d2_interior(u[p - 2 * s], u[p - s], u[p], u[p + s], u[p + 2 * s], b[p - 2 * s], b[p - s], b[p],
-                           b[p + s], b[p + 2 * s]);
-    }
-    return d2_closure(u, b, p, i, n, s);
-}
-
-// D1 (times h) on a boundary row (i < 4 or i >= n-4), reading global memory.
-__device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, int i, int n, int s) {
-    const bool lo = i < 4;
-    const int r = lo ? i : n - 1 - i;
-    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
-    const int step = lo ? s : -s;
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 6; j++) acc += c_d1b[r][j] * u[base + j * step];
-    return lo ? acc : -acc;
-}
-
-// D1 (times h) at axis index i reading global memory.
-__device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, int i, int n, int s) {
-    if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);
-    return d1_closure(u, p, i, n, s);
-}
-
 // ------------------------------------------------------------------------------------
 // Closure rows with compile-time coefficients: row R keeps only its non-zero entries
 // of the sparse tables, so a wave whose lanes sit on different rows runs each row's
@@ -167,16 +112,65 @@ __device__ __forceinline__ float d1_rows(int r, const U& u) {
     }
 }
 
-// Closure rows read from global memory, out of line: the y-marching kernels take
-// these wave-uniform branches on a few x rows and y planes only, and keeping the
-// code out of the loop body keeps its register allocation at the interior path's.
-// base = offset of the face point, st = +/- stride inward.
-__device__ __attribute__((noinline)) float d2_rows_g(const float* __restrict__ u, const float* __restrict__ b, int base,
-                                                     int st, int r) {
-    return d2_rows(r, [&](int j) { return u[base + j * st]; }, [&](int k) { return b[base + k * st]; });
+// Closure rows with the coefficient table in constant memory, all rows in one pass:
+// the form for lanes of one wave sitting on different rows (the z axis), where a
+// per-row branch would run the rows one after another, each waiting for its loads.
+__device__ __forceinline__ float d2_closure_tab(const float* __restrict__ u, const float* __restrict__ b, int base,
+                                                int step, int r) {
+    float bk[8], uk[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        bk[k] = b[base + k * step];
+        uk[k] = u[base + k * step];
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float w = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) w += c_d2b[r][j][k] * bk[k];
+        acc += w * uk[j];
+    }
+    return acc;
 }
-__device__ __attribute__((noinline)) float d1_rows_g(const float* __restrict__ f, int base, int st, int r) {
-    return d1_rows(r, [&](int j) { return f[base + j * st]; });
+__device__ __forceinline__ float d1_closure_tab(const float* __restrict__ u, int base, int step, int r) {
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 6; j++) acc += c_d1b[r][j] * u[base + j * step];
+    return acc;
+}
+
+// D2(b) (times h^2) at axis index i (any row) reading global memory: the field and
+// coefficient at point p, axis length n, element stride s. Boundary rows (i < 6 or
+// i >= n-6): along z (s == 1, rows differ across a wave) the one-pass table form,
+// along x / y (one row per wave) the compile-time rows.
+__device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
+                                          int s) {
+    if (i >= 6 && i < n - 6) {
+        return d2_interior(u[p - 2 * s], u[p - s], u[p], u[p + s], u[p + 2 * s], b[p - 2 * s], b[p - s], b[p],
+                           b[p + s], b[p + 2 * s]);
+    }
+    const bool lo = i < 6;
+    const int r = lo ? i : n - 1 - i;
+    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
+    const int step = lo ? s : -s;
+    if (s == 1) return d2_closure_tab(u, b, base, step, r);
+    return d2_rows(r, [&](int j) { return u[base + j * step]; }, [&](int k) { return b[base + k * step]; });
+}
+// D1 (times h) on a boundary row (i < 4 or i >= n-4), reading global memory (row
+// forms as for d2_point).
+__device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, int i, int n, int s) {
+    const bool lo = i < 4;
+    const int r = lo ? i : n - 1 - i;
+    const int base = lo ? p - i * s : p + (n - 1 - i) * s;
+    const int step = lo ? s : -s;
+    const float v = s == 1 ? d1_closure_tab(u, base, step, r) : d1_rows(r, [&](int j) { return u[base + j * step]; });
+    return lo ? v : -v;
+}
+// D1 (times h) at axis index i reading global memory.
+__device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, int i, int n, int s) {
+    if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);
+    return d1_closure(u, p, i, n, s);
 }
 
 // Outward boundary derivative S (times h) at a face point p, `inward` = +/- stride.
@@ -305,16 +299,6 @@ __device__ __forceinline__ bool shell_point(int t, int w, int nx, int ny, int nz
     return false;
 }
 
-// The two 6-deep z slabs (z < 6 and z >= nz-6) of every (x, y) column, 12 points per
-// column with consecutive threads on consecutive z. Returns false past the end.
-__device__ __forceinline__ bool zslab_point(int t, int nx, int ny, int nz, int& ix, int& iy, int& iz) {
-    const int col = t / 12, k = t - col * 12;
-    if (col >= nx * ny) return false;
-    iy = col / nx;
-    ix = col - iy * nx;
-    iz = k < 6 ? k : nz - 12 + k;
-    return true;
-}
 __host__ __device__ inline int shell_count(int w, int nx, int ny, int nz) {
     return 2 * w * nx * nz + (ny - 2 * w) * 2 * w * nz + (ny - 2 * w) * (nx - 2 * w) * 2 * w;
 }

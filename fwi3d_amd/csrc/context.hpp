@@ -67,8 +67,11 @@ struct fwi3d_ctx {
     long long snap_n = 0;
     bool snap_valid = false;
     int snap_dims[3] = {0, 0, 0};
-    // profiling: [0] time-step kernel (z in [6, nz-6)), [1] flux pre-pass (elastic/VTI),
-    // [2] face SATs, [3] z-slab points
+    // profiling: [0] interior time-step kernel, [1] flux pre-pass (elastic/VTI),
+    // [2] boundary-shell time-step kernel, [3] reserved
+    // tiling of the y-marching kernels (fwi3d_tune)
+    int tune_ychunk = 0;  // y planes per workgroup (0: default 8)
+    int tune_tx = 4;      // x rows per workgroup (4 or 8)
     bool prof = false;
     double prof_ms[4] = {0, 0, 0, 0};
     long long prof_n[4] = {0, 0, 0, 0};

@@ -1,17 +1,16 @@
 // Elastic isotropic and VTI kernels (gfx950): one time step of
 // nl_we_op_e::propagate (src/cpp/we_op.cpp:646-836) / nl_we_op_vti::propagate
-// (we_op.cpp:1290-1497) in four launches:
+// (we_op.cpp:1290-1497) in three launches:
 //
 //   elastic_flux      (whole grid) first derivatives of the displacement -> normal
 //                     stress T (lambda*div u; VTI: three c11c/c12/c13 combinations) and
 //                     the six shear fluxes F_ac = G * D_c u_a (a != c; G = mu, or c66
 //                     for the x-y coupling in VTI), written to HBM.
-//   elastic_step      the points 6+ cells from the z faces: 9x D2(b) u_c + D_c T_c
-//                     + D_a F_ac (interior stencils; SBP closure rows for x rows / y
-//                     planes near a face) from LDS x-z planes and y register queues,
-//                     leapfrog update, absorbing scaling, taper.
-//   elastic_zslab     the same for the 6-deep z slabs, one thread per point.
-//   elastic_faces     adds the traction SATs on the six faces.
+//   elastic_interior  points 6+ cells inside every face: 9x D2(b) u_c + D_c T_c +
+//                     D_a F_ac from LDS x-z planes and y register queues, leapfrog
+//                     update, cos^2 taper.
+//   elastic_boundary  the 6-cell shell: the same terms with SBP closure rows, the
+//                     traction SATs, the update, the absorbing scaling and the taper.
 //
 // The displacement components are stored one after the other ([3][y][x][z]).
 // Flux layout in the aux buffer: [T (1 or 3)][F_yx F_zx F_xy F_zy F_xz F_yz].
@@ -46,7 +45,6 @@ struct EArgs {
     float dt, dt2;
     int prev_tapered;
     int ychunk, ntz, ntx, nyc;
-    int zlo, zhi;  // z range of the y-marching kernel (z tiles start at zlo)
 };
 
 // ------------------------------------------------------------------------------------
@@ -67,8 +65,8 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
     const int lz = threadIdx.x, lx = threadIdx.y;
-    const int z0 = A.zlo + tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
-    const bool valid = (iz < A.zhi) && (ix < nx);
+    const int z0 = tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
+    const bool valid = (iz < nz) && (ix < nx);
     const int y0 = cy * A.ychunk, y1 = min(ny, y0 + A.ychunk);
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
@@ -164,7 +162,154 @@ __device__ __forceinline__ float d2_factor(int c, int a) {
 }
 
 // ------------------------------------------------------------------------------------
-// Coefficients of the traction SATs and the absorbing scaling.
+// Pass 2, interior points.
+// LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_zx F_xy F_zy F_xz.
+// Register queues (y neighbours): u (3), b-fields, T_y, F_yx, F_yz.
+// b-fields: elastic {mu}; VTI {mu, c66, c11b}.
+template <int TX, bool VTI>
+__global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
+    constexpr int TZ = 64;
+    constexpr int NB = VTI ? 3 : 1;   // D2 coefficient fields
+    constexpr int NT = VTI ? 3 : 1;   // normal-stress fields
+    constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
+    constexpr int NPL = PF + 4;
+    constexpr int NQ = 3 + NB + 3;
+    __shared__ float s[NPL][TX + 4][TZ + 4];
+    const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
+    const float dx = A.g.dx, dy = A.g.dy, dz = A.g.dz;
+    const float* __restrict__ curr = A.curr;
+    const float* __restrict__ prev = A.prev;
+    float* __restrict__ next = A.next;
+    const float* __restrict__ rhof = A.rho;
+    const float* __restrict__ tz2 = A.tz2;
+    const float* __restrict__ tx2 = A.tx2;
+    const float* __restrict__ ty2 = A.ty2;
+    const float dt2 = A.dt2;
+    const bool ptap = A.prev_tapered != 0;
+    const float* bf[3] = {A.mu, VTI ? A.c66 : A.mu, VTI ? A.c11b : A.mu};
+    const float* Ff = A.aux + NT * n3;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
+    const int lz = threadIdx.x, lx = threadIdx.y;
+    const int z0 = tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
+    const int y0 = cy * A.ychunk, y1 = min(ny, y0 + A.ychunk);
+    const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
+    const int col = cx * sx + cz;
+    const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
+    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    HaloMap<TX> H;
+    H.init(lx, lz, x0, z0, nx, nz, sx);
+
+    // queue sources: u(3), b(NB), T_y, F_yx, F_yz
+    const float* qsrc[NQ];
+#pragma unroll
+    for (int c = 0; c < 3; c++) qsrc[c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) qsrc[3 + k] = bf[k];
+    qsrc[3 + NB] = A.aux + (VTI ? n3 : 0);
+    qsrc[4 + NB] = Ff + FX_YX * n3;
+    qsrc[5 + NB] = Ff + FX_YZ * n3;
+    // LDS plane sources
+    const float* psrc[NPL];
+#pragma unroll
+    for (int c = 0; c < 3; c++) psrc[PU + c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) psrc[PB + k] = bf[k];
+    psrc[PT] = A.aux;
+    if (VTI) psrc[PT + 1] = A.aux + 2 * n3;
+    psrc[PF + 0] = Ff + FX_ZX * n3;
+    psrc[PF + 1] = Ff + FX_XY * n3;
+    psrc[PF + 2] = Ff + FX_ZY * n3;
+    psrc[PF + 3] = Ff + FX_XZ * n3;
+
+    float q[NQ][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
+#pragma unroll
+    for (int k = 0; k < NQ; k++)
+#pragma unroll
+        for (int o = 0; o < 5; o++) q[k][o] = ldy(qsrc[k], y0 - 2 + o);
+    const float ih2[3] = {1.0f / (dx * dx), 1.0f / (dy * dy), 1.0f / (dz * dz)};
+    // values of the next plane to stage (centres of the flux planes, halos of all
+    // planes), loaded one iteration ahead of their use
+    float pc[NPL], hx[NPL], hz[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; k++) pc[k] = hx[k] = hz[k] = 0.f;
+    auto prefetch = [&](int y) {
+        const int pl = min(y, ny - 1) * sy;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            const float* __restrict__ f = psrc[k];
+            if (k >= 3 + NB) pc[k] = f[pl + col];
+            if (H.has_x) hx[k] = f[pl + H.xsrc];
+            if (H.has_z) hz[k] = f[pl + H.zsrc];
+        }
+    };
+    prefetch(y0);
+
+    for (int iy = y0; iy < y1; iy++) {
+#pragma unroll
+        for (int k = 0; k < NQ; k++) q[k][5] = ldy(qsrc[k], iy + 3);
+        const int pl = iy * sy;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            s[k][lx + 2][lz + 2] = (k < 3 + NB) ? q[k][2] : pc[k];
+            if (H.has_x) s[k][H.xrow][H.xcol] = hx[k];
+            if (H.has_z) s[k][H.zrow][H.zcol] = hz[k];
+        }
+        prefetch(iy + 1);
+        __syncthreads();
+        if (comp_xz && iy >= 6 && iy < ny - 6) {
+            const int p = pl + col;
+            auto SX = [&](int k, int o) { return s[k][lx + 2 + o][lz + 2]; };
+            auto SZ = [&](int k, int o) { return s[k][lx + 2][lz + 2 + o]; };
+            auto d1x = [&](int k) { return D1C1 * (SX(k, 1) - SX(k, -1)) + D1C2 * (SX(k, 2) - SX(k, -2)); };
+            auto d1z = [&](int k) { return D1C1 * (SZ(k, 1) - SZ(k, -1)) + D1C2 * (SZ(k, 2) - SZ(k, -2)); };
+            auto d1y = [&](int k) { return D1C1 * (q[k][3] - q[k][1]) + D1C2 * (q[k][4] - q[k][0]); };
+            auto d2 = [&](int c, int kb, int a) -> float {
+                if (a == 0)
+                    return d2_interior(SX(PU + c, -2), SX(PU + c, -1), SX(PU + c, 0), SX(PU + c, 1), SX(PU + c, 2),
+                                       SX(PB + kb, -2), SX(PB + kb, -1), SX(PB + kb, 0), SX(PB + kb, 1), SX(PB + kb, 2));
+                if (a == 1)
+                    return d2_interior(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4], q[3 + kb][0], q[3 + kb][1],
+                                       q[3 + kb][2], q[3 + kb][3], q[3 + kb][4]);
+                return d2_interior(SZ(PU + c, -2), SZ(PU + c, -1), SZ(PU + c, 0), SZ(PU + c, 1), SZ(PU + c, 2),
+                                   SZ(PB + kb, -2), SZ(PB + kb, -1), SZ(PB + kb, 0), SZ(PB + kb, 1), SZ(PB + kb, 2));
+            };
+            const float rho = rhof[p];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float acc = (d2_factor<VTI>(c, 0) * ih2[0]) * d2(c, d2_bfield<VTI>(c, 0), 0);
+                acc = acc + (d2_factor<VTI>(c, 1) * ih2[1]) * d2(c, d2_bfield<VTI>(c, 1), 1);
+                acc = acc + (d2_factor<VTI>(c, 2) * ih2[2]) * d2(c, d2_bfield<VTI>(c, 2), 2);
+                // normal stress and shear fluxes (we_op.cpp:660-688)
+                if (c == 0) {
+                    acc = acc + d1x(PT) / dx;
+                    acc = acc + d1y(4 + NB) / dy;     // D_y F_yx
+                    acc = acc + d1z(PF + 0) / dy;     // D_z F_zx (spacing dy as in the reference)
+                } else if (c == 1) {
+                    acc = acc + d1y(3 + NB) / dy;     // D_y T_y
+                    acc = acc + d1x(PF + 1) / dx;     // D_x F_xy
+                    acc = acc + d1z(PF + 2) / dz;     // D_z F_zy
+                } else {
+                    acc = acc + d1z(VTI ? PT + 1 : PT) / dz;
+                    acc = acc + d1x(PF + 3) / dx;     // D_x F_xz
+                    acc = acc + d1y(5 + NB) / dy;     // D_y F_yz
+                }
+                float pv = prev[c * n3 + p];
+                if (ptap) pv = taper_apply(pv, tz2, tx2, ty2, ix, iy, iz);
+                const float nv = dt2 * acc / rho + 2.f * q[c][2] - pv;
+                next[c * n3 + p] = taper_apply(nv, tz2, tx2, ty2, ix, iy, iz);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Pass 2, boundary shell: one thread per point, every operand from global memory.
 template <bool VTI>
 struct ECoef {
     const float *lam, *mu, *rho, *c13, *eps, *c66;
@@ -226,283 +371,79 @@ __device__ __forceinline__ float e_scale(const Geom& g, const int* bc, const ECo
     return any ? 1.f + sc : 1.f;
 }
 
-// Closure-row bookkeeping of one axis (see acoustic.hpp): D2 row, D1 row, side.
-// (Rows is defined there.)
-
-// ------------------------------------------------------------------------------------
-// Pass 2, the points 6+ cells from the z faces (z in [6, nz-6)), every x and y: 9x
-// D2(b) u_c + D_c T_c + D_a F_ac, the leapfrog update, the absorbing scaling and the
-// taper. Interior stencils come from LDS x-z planes and y register queues; the SBP
-// closure rows of x rows and y planes near a face from L2-resident global memory
-// (wave- and iteration-uniform branches). The z slabs are elastic_zslab's; the face
-// SATs follow in elastic_faces.
-// LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_zx F_xy F_zy F_xz.
-// Register queues (y neighbours): u (3), b-fields, T_y, F_yx, F_yz.
-// b-fields: elastic {mu}; VTI {mu, c66, c11b}.
-template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX, 3) void elastic_step(EArgs A) {
-    constexpr int TZ = 64;
-    constexpr int NB = VTI ? 3 : 1;   // D2 coefficient fields
-    constexpr int NT = VTI ? 3 : 1;   // normal-stress fields
-    constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
-    constexpr int NPL = PF + 4;
-    constexpr int NQ = 3 + NB + 3;
-    __shared__ float s[NPL][TX + 4][TZ + 4];
-    const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
-    const float dx = A.g.dx, dy = A.g.dy, dz = A.g.dz;
-    const float* __restrict__ curr = A.curr;
-    const float* __restrict__ prev = A.prev;
-    float* __restrict__ next = A.next;
-    const float* __restrict__ rhof = A.rho;
-    const float* __restrict__ tz2 = A.tz2;
-    const float* __restrict__ tx2 = A.tx2;
-    const float* __restrict__ ty2 = A.ty2;
-    const float dt2 = A.dt2;
-    const bool ptap = A.prev_tapered != 0;
-    const float* Ff = A.aux + NT * n3;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
-    // blockDim.x == 64: each wave is one x row, so lx is wave-uniform
-    const int lz = threadIdx.x, lx = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int z0 = A.zlo + tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
-    const int y0 = cy * A.ychunk, y1 = min(ny, y0 + A.ychunk);
-    const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
-    const int col = cx * sx + cz;
-    const bool valid = ix < nx && iz < A.zhi;
-    const Rows RX(ix, nx);
-    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
-    HaloMap<TX> H;
-    H.init(lx, lz, x0, z0, nx, nz, sx);
-
-    // field pointers: u(3), b(NB), T_y, F_yx, F_yz (queues); plane sources
-    auto bfield = [&](int k) -> const float* { return k == 0 ? A.mu : (k == 1 ? (VTI ? A.c66 : A.mu) : (VTI ? A.c11b : A.mu)); };
-    const float* qsrc[NQ];
-#pragma unroll
-    for (int c = 0; c < 3; c++) qsrc[c] = curr + c * n3;
-#pragma unroll
-    for (int k = 0; k < NB; k++) qsrc[3 + k] = bfield(k);
-    qsrc[3 + NB] = A.aux + (VTI ? n3 : 0);
-    qsrc[4 + NB] = Ff + FX_YX * n3;
-    qsrc[5 + NB] = Ff + FX_YZ * n3;
-    const float* psrc[NPL];
-#pragma unroll
-    for (int c = 0; c < 3; c++) psrc[PU + c] = curr + c * n3;
-#pragma unroll
-    for (int k = 0; k < NB; k++) psrc[PB + k] = bfield(k);
-    psrc[PT] = A.aux;
-    if (VTI) psrc[PT + 1] = A.aux + 2 * n3;
-    psrc[PF + 0] = Ff + FX_ZX * n3;
-    psrc[PF + 1] = Ff + FX_XY * n3;
-    psrc[PF + 2] = Ff + FX_ZY * n3;
-    psrc[PF + 3] = Ff + FX_XZ * n3;
-
-    float q[NQ][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
-#pragma unroll
-    for (int k = 0; k < NQ; k++)
-#pragma unroll
-        for (int o = 0; o < 5; o++) q[k][o] = ldy(qsrc[k], y0 - 2 + o);
-    const float ih2[3] = {1.0f / (dx * dx), 1.0f / (dy * dy), 1.0f / (dz * dz)};
-    float pc[NPL], hx[NPL], hz[NPL];
-#pragma unroll
-    for (int k = 0; k < NPL; k++) pc[k] = hx[k] = hz[k] = 0.f;
-    auto prefetch = [&](int y) {
-        const int pl = min(y, ny - 1) * sy;
-#pragma unroll
-        for (int k = 0; k < NPL; k++) {
-            const float* __restrict__ f = psrc[k];
-            if (k >= 3 + NB) pc[k] = f[pl + col];
-            if (H.has_x) hx[k] = f[pl + H.xsrc];
-            if (H.has_z) hz[k] = f[pl + H.zsrc];
-        }
-    };
-    prefetch(y0);
-
-    for (int iy = y0; iy < y1; iy++) {
-#pragma unroll
-        for (int k = 0; k < NQ; k++) q[k][5] = ldy(qsrc[k], iy + 3);
-        const int pl = iy * sy;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NPL; k++) {
-            s[k][lx + 2][lz + 2] = (k < 3 + NB) ? q[k][2] : pc[k];
-            if (H.has_x) s[k][H.xrow][H.xcol] = hx[k];
-            if (H.has_z) s[k][H.zrow][H.zcol] = hz[k];
-        }
-        prefetch(iy + 1);
-        __syncthreads();
-        if (valid) {
-            const int p = pl + col;
-            const Rows RY(iy, ny);
-            auto SX = [&](int k, int o) { return s[k][lx + 2 + o][lz + 2]; };
-            auto SZ = [&](int k, int o) { return s[k][lx + 2][lz + 2 + o]; };
-            // closure rows of x rows / y planes near a face: from the face inward, global (L2)
-            const int bx = pl + (RX.lo ? 0 : (nx - 1) * sx) + cz, stx = RX.lo ? sx : -sx;
-            const int by = (RY.lo ? 0 : (ny - 1) * sy) + col, sty = RY.lo ? sy : -sy;
-            // D1 of a flux field along x / y / z (times h)
-            auto d1x = [&](int k, const float* __restrict__ f) -> float {
-                if (RX.r1 < 0) return D1C1 * (SX(k, 1) - SX(k, -1)) + D1C2 * (SX(k, 2) - SX(k, -2));
-                const float v = d1_rows_g(f, bx, stx, RX.r1);
-                return RX.lo ? v : -v;
-            };
-            auto d1y = [&](int k, const float* __restrict__ f) -> float {
-                if (RY.r1 < 0) return D1C1 * (q[k][3] - q[k][1]) + D1C2 * (q[k][4] - q[k][0]);
-                const float v = d1_rows_g(f, by, sty, RY.r1);
-                return RY.lo ? v : -v;
-            };
-            auto d1z = [&](int k, const float* __restrict__) -> float {
-                return D1C1 * (SZ(k, 1) - SZ(k, -1)) + D1C2 * (SZ(k, 2) - SZ(k, -2));
-            };
-            // D2(b) of component c along axis a (times h^2)
-            auto d2 = [&](int c, int kb, int a) -> float {
-                const float* __restrict__ u = curr + c * n3;
-                const float* __restrict__ b = bfield(kb);
-                if (a == 0) {
-                    if (RX.r2 < 0)
-                        return d2_interior(SX(PU + c, -2), SX(PU + c, -1), SX(PU + c, 0), SX(PU + c, 1), SX(PU + c, 2),
-                                           SX(PB + kb, -2), SX(PB + kb, -1), SX(PB + kb, 0), SX(PB + kb, 1), SX(PB + kb, 2));
-                    return d2_rows_g(u, b, bx, stx, RX.r2);
-                }
-                if (a == 1) {
-                    if (RY.r2 < 0)
-                        return d2_interior(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4], q[3 + kb][0], q[3 + kb][1],
-                                           q[3 + kb][2], q[3 + kb][3], q[3 + kb][4]);
-                    return d2_rows_g(u, b, by, sty, RY.r2);
-                }
-                return d2_interior(SZ(PU + c, -2), SZ(PU + c, -1), SZ(PU + c, 0), SZ(PU + c, 1), SZ(PU + c, 2),
-                                   SZ(PB + kb, -2), SZ(PB + kb, -1), SZ(PB + kb, 0), SZ(PB + kb, 1), SZ(PB + kb, 2));
-            };
-            const float rho = rhof[p];
-            const bool face = ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float acc = (d2_factor<VTI>(c, 0) * ih2[0]) * d2(c, d2_bfield<VTI>(c, 0), 0);
-                acc = acc + (d2_factor<VTI>(c, 1) * ih2[1]) * d2(c, d2_bfield<VTI>(c, 1), 1);
-                acc = acc + (d2_factor<VTI>(c, 2) * ih2[2]) * d2(c, d2_bfield<VTI>(c, 2), 2);
-                // normal stress and shear fluxes (we_op.cpp:660-688)
-                if (c == 0) {
-                    acc = acc + d1x(PT, A.aux) / dx;
-                    acc = acc + d1y(4 + NB, Ff + FX_YX * n3) / dy;  // D_y F_yx
-                    acc = acc + d1z(PF + 0, Ff + FX_ZX * n3) / dy;  // D_z F_zx (spacing dy as in the reference)
-                } else if (c == 1) {
-                    acc = acc + d1y(3 + NB, A.aux + (VTI ? n3 : 0)) / dy;  // D_y T_y
-                    acc = acc + d1x(PF + 1, Ff + FX_XY * n3) / dx;          // D_x F_xy
-                    acc = acc + d1z(PF + 2, Ff + FX_ZY * n3) / dz;          // D_z F_zy
-                } else {
-                    acc = acc + d1z(VTI ? PT + 1 : PT, A.aux + (VTI ? 2 * n3 : 0)) / dz;
-                    acc = acc + d1x(PF + 3, Ff + FX_XZ * n3) / dx;  // D_x F_xz
-                    acc = acc + d1y(5 + NB, Ff + FX_YZ * n3) / dy;  // D_y F_yz
-                }
-                float pv = prev[c * n3 + p];
-                if (ptap) pv = taper_apply(pv, tz2, tx2, ty2, ix, iy, iz);
-                float nv = dt2 * acc / rho + 2.f * q[c][2] - pv;
-                if (face) {
-                    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-                    nv = nv / e_scale<VTI>(A.g, A.bc, M, p, ix, iy, iz, c, A.dt);
-                }
-                next[c * n3 + p] = taper_apply(nv, tz2, tx2, ty2, ix, iy, iz);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < NQ; k++) {
-            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
-        }
-    }
-}
-
-// The z slabs (z < 6 or z >= nz-6): one thread per (point, component = blockIdx.y),
-// operands from global memory (closure rows along z, and along x / y near those
-// faces), update, scaling, taper.
 template <bool VTI>
-__global__ __launch_bounds__(256) void elastic_zslab(EArgs A) {
+__global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
     const Geom g = A.g;
     int ix, iy, iz;
-    if (!zslab_point(blockIdx.x * blockDim.x + threadIdx.x, g.nx, g.ny, g.nz, ix, iy, iz)) return;
-    const int c = blockIdx.y;
+    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    const int c = blockIdx.y;  // displacement component of this thread
     constexpr int NT = VTI ? 3 : 1;
     const int n3 = g.n;
     const float* __restrict__ curr = A.curr;
+    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
+    // selects instead of small arrays: c is a runtime value and indexed arrays would spill
     auto BF = [&](int k) -> const float* { return k == 0 ? A.mu : (k == 1 ? (VTI ? A.c66 : A.mu) : (VTI ? A.c11b : A.mu)); };
     auto IDX = [&](int a) { return a == 0 ? ix : (a == 1 ? iy : iz); };
     const float* F = A.aux + NT * n3;
     const int p = g.at(ix, iy, iz);
     auto D = [&](const float* f, int a) { return d1_point(f, p, IDX(a), ax_len(g, a), ax_stride(g, a)); };
-    const float* u = curr + c * n3;
-    float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2_point(u, BF(d2_bfield<VTI>(c, 0)), p, ix, g.nx, g.sx);
-    acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2_point(u, BF(d2_bfield<VTI>(c, 1)), p, iy, g.ny, g.sy);
-    acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2_point(u, BF(d2_bfield<VTI>(c, 2)), p, iz, g.nz, 1);
-    if (c == 0) {
-        acc = acc + D(A.aux, 0) / g.dx;
-        acc = acc + D(F + FX_YX * n3, 1) / g.dy;
-        acc = acc + D(F + FX_ZX * n3, 2) / g.dy;  // spacing dy as in the reference
-    } else if (c == 1) {
-        acc = acc + D(A.aux + (VTI ? n3 : 0), 1) / g.dy;
-        acc = acc + D(F + FX_XY * n3, 0) / g.dx;
-        acc = acc + D(F + FX_ZY * n3, 2) / g.dz;
-    } else {
-        acc = acc + D(A.aux + (VTI ? 2 * n3 : 0), 2) / g.dz;
-        acc = acc + D(F + FX_XZ * n3, 0) / g.dx;
-        acc = acc + D(F + FX_YZ * n3, 1) / g.dy;
-    }
-    float pv = A.prev[c * n3 + p];
-    if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-    float nv = A.dt2 * acc / A.rho[p] + 2.f * u[p] - pv;
-    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-    nv = nv / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, c, A.dt);
-    A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-}
-
-// Traction SATs of the faces through each surface point (reference face order),
-// added to the updated field: next += taper(dt^2 sat / rho / scale). One thread per
-// (surface point, component = blockIdx.y).
-template <bool VTI>
-__global__ __launch_bounds__(256) void elastic_faces(EArgs A) {
-    const Geom g = A.g;
-    int ix, iy, iz;
-    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 1, g.nx, g.ny, g.nz, ix, iy, iz)) return;
-    const int c = blockIdx.y;
-    const int n3 = g.n;
-    const float* __restrict__ curr = A.curr;
-    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-    auto IDX = [&](int a) { return a == 0 ? ix : (a == 1 ? iy : iz); };
-    const int p = g.at(ix, iy, iz);
-    auto D = [&](const float* f, int a) { return d1_point(f, p, IDX(a), ax_len(g, a), ax_stride(g, a)); };
-    float pv = A.prev[c * n3 + p];
-    if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
-    float sat = 0.f;
-    bool any = false;
-#pragma unroll
-    for (int f = 0; f < 6; f++) {
-        const int bc = A.bc[f];
-        if (bc != 1 && bc != 2) continue;
-        const int nax = f < 2 ? 2 : (f < 4 ? 0 : 1);
-        const bool hi = f & 1;
-        const int len = ax_len(g, nax);
-        if (!(hi ? IDX(nax) == len - 1 : IDX(nax) == 0)) continue;
-        any = true;
-        const int t1 = nax == 2 ? 0 : (nax == 0 ? 1 : 0);
-        const int t2 = nax == 2 ? 1 : 2;
-        const float sg = hi ? 1.f : -1.f;
-        const float hn = ax_h(g, nax);
-        const int inward = hi ? -ax_stride(g, nax) : ax_stride(g, nax);
-        const float adh = 1.0f / (hn * SAT_H0);
-        float term;
-        if (c == nax) {
-            const float tang = M.cnorm(nax, t1, p) * (D(curr + t1 * n3, t1) / ax_h(g, t1)) +
-                               M.cnorm(nax, t2, p) * (D(curr + t2 * n3, t2) / ax_h(g, t2)) +
-                               M.cnorm(nax, nax, p) * (D(curr + nax * n3, nax) / hn);
-            term = sg * tang + M.knn(nax, p) * sat_s(curr + nax * n3, p, inward) / hn;
-            if (bc == 2) term -= M.pimp(nax, p) * pv / A.dt;
+    const bool face = (iz == 0) | (iz == g.nz - 1) | (ix == 0) | (ix == g.nx - 1) | (iy == 0) | (iy == g.ny - 1);
+    const float rho = M.rho[p];
+    {
+        const float* u = curr + c * n3;
+        float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2_point(u, BF(d2_bfield<VTI>(c, 0)), p, ix, g.nx, g.sx);
+        acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2_point(u, BF(d2_bfield<VTI>(c, 1)), p, iy, g.ny, g.sy);
+        acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2_point(u, BF(d2_bfield<VTI>(c, 2)), p, iz, g.nz, 1);
+        if (c == 0) {
+            acc = acc + D(A.aux, 0) / g.dx;
+            acc = acc + D(F + FX_YX * n3, 1) / g.dy;
+            acc = acc + D(F + FX_ZX * n3, 2) / g.dy;  // spacing dy as in the reference
+        } else if (c == 1) {
+            acc = acc + D(A.aux + (VTI ? n3 : 0), 1) / g.dy;
+            acc = acc + D(F + FX_XY * n3, 0) / g.dx;
+            acc = acc + D(F + FX_ZY * n3, 2) / g.dz;
         } else {
-            const float G = M.gshear(nax, c, p);
-            term = sg * G * (D(curr + nax * n3, c) / ax_h(g, c)) + G * sat_s(curr + c * n3, p, inward) / hn;
-            if (bc == 2) term -= 0.5f * sqrtf(M.mu[p] * M.rho[p]) * pv / A.dt;
+            acc = acc + D(A.aux + (VTI ? 2 * n3 : 0), 2) / g.dz;
+            acc = acc + D(F + FX_XZ * n3, 0) / g.dx;
+            acc = acc + D(F + FX_YZ * n3, 1) / g.dy;
         }
-        sat -= adh * term;
+        float pv = A.prev[c * n3 + p];
+        if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+        if (face) {
+            // traction SATs of the faces through p (reference face order)
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                const int bc = A.bc[f];
+                if (bc != 1 && bc != 2) continue;
+                const int nax = f < 2 ? 2 : (f < 4 ? 0 : 1);
+                const bool hi = f & 1;
+                const int len = ax_len(g, nax);
+                if (!(hi ? IDX(nax) == len - 1 : IDX(nax) == 0)) continue;
+                const int t1 = nax == 2 ? 0 : (nax == 0 ? 1 : 0);
+                const int t2 = nax == 2 ? 1 : 2;
+                const float sg = hi ? 1.f : -1.f;
+                const float hn = ax_h(g, nax);
+                const int inward = hi ? -ax_stride(g, nax) : ax_stride(g, nax);
+                const float adh = 1.0f / (hn * SAT_H0);
+                float term;
+                if (c == nax) {
+                    const float tang = M.cnorm(nax, t1, p) * (D(curr + t1 * n3, t1) / ax_h(g, t1)) +
+                                       M.cnorm(nax, t2, p) * (D(curr + t2 * n3, t2) / ax_h(g, t2)) +
+                                       M.cnorm(nax, nax, p) * (D(curr + nax * n3, nax) / hn);
+                    term = sg * tang + M.knn(nax, p) * sat_s(curr + nax * n3, p, inward) / hn;
+                    if (bc == 2) term -= M.pimp(nax, p) * pv / A.dt;
+                } else {
+                    const float G = M.gshear(nax, c, p);
+                    term = sg * G * (D(curr + nax * n3, c) / ax_h(g, c)) + G * sat_s(curr + c * n3, p, inward) / hn;
+                    if (bc == 2) term -= 0.5f * sqrtf(M.mu[p] * M.rho[p]) * pv / A.dt;
+                }
+                acc -= adh * term;
+            }
+        }
+        float nv = A.dt2 * acc / rho + 2.f * u[p] - pv;
+        if (face) nv = nv / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, c, A.dt);
+        A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
     }
-    if (!any) return;
-    float add = A.dt2 * sat / M.rho[p];
-    add = add / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, c, A.dt);
-    A.next[c * n3 + p] += taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
 }
 
 // Source injection, all three displacement components (blockIdx.y):

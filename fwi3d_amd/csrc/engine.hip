@@ -175,15 +175,14 @@ Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
     return Tapers{dtz, dtx, dty};
 }
 
-// Tiling of the y-marching kernels: 64(z) x TX(x) column tiles over z in [zlo, zhi),
-// y in chunks.
+// Tiling of the y-marching kernels: 64(z) x TX(x) column tiles, y in chunks of
+// `ychunk` planes (context tuning; default 8: many short chunks balance better than
+// fewer long ones, measured on MI355X at 200^3).
 template <class Args>
-int tile_grid(Args& A, const Geom& g, int TX, int zlo, int zhi) {
-    A.zlo = zlo;
-    A.zhi = zhi;
-    A.ntz = (zhi - zlo + 63) / 64;
+int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX) {
+    A.ntz = (g.nz + 63) / 64;
     A.ntx = (g.nx + TX - 1) / TX;
-    A.ychunk = std::max(4, std::min(g.ny, 16));
+    A.ychunk = std::max(4, std::min(g.ny, c->tune_ychunk > 0 ? c->tune_ychunk : 8));
     A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;
 }
@@ -270,15 +269,14 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     float* curr = F + n;
     float* next = F + 2 * n;
 
-    constexpr int TX = 4;
+    const int TX = c->tune_tx;
     AcuArgs A{};
     A.K = M; A.b = M + n; A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2; A.g = g;
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
     A.dt = dt; A.dt2 = dt * dt;
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
-    const int nblk = tile_grid(A, g, TX, 6, g.nz - 6);      // z slabs: acoustic_zslab
-    const int nslab = 12 * g.nx * g.ny;
-    const int nshell = shell_count(4, g.nx, g.ny, g.nz);  // points within 4 cells of a face (SATs)
+    const int nblk = tile_grid(c, A, g, TX);
+    const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
 
     const int nthr = 256;
     const int nbi = (I.npts * 27 + nthr - 1) / nthr;
@@ -310,9 +308,11 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        timed(c, 0, [&] { hipLaunchKernelGGL(acoustic_step<TX>, dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-        timed(c, 3, [&] { hipLaunchKernelGGL(acoustic_zslab, dim3((nslab + 255) / 256), dim3(256), 0, c->stream, A); });
-        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_faces, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+        timed(c, 0, [&] {
+            if (TX == 8) hipLaunchKernelGGL(acoustic_interior<8>, dim3(nblk), dim3(64, 8), 0, c->stream, A);
+            else hipLaunchKernelGGL(acoustic_interior<4>, dim3(nblk), dim3(64, 4), 0, c->stream, A);
+        });
+        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_boundary, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
         if (I.npts > 0)
             hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, next, A, I, d_src, nt, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
@@ -332,6 +332,14 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
     if (i < n) c11b[i] = 2.f * (1.f + 2.f * eps[i]) * mu[i];
 }
 
+// One elastic/VTI time step: flux pre-pass, interior y-march, boundary shell.
+template <int TX, bool VTI>
+void step_elastic(fwi3d_ctx* c, const EArgs& A, int nblk, int nshell) {
+    timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, VTI>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+    timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, VTI>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+    timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<VTI>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+}
+
 void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, const float* d_src, float* d_rcv,
                        const fwi3d_injector* inj, const fwi3d_injector* ext, double* d_grad, const fwi3d_param* par,
                        const Geom& g) {
@@ -341,7 +349,7 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int nT = vti ? 3 : 1;
     c->fields.ensure(9 * n * sizeof(float));
     c->model.ensure(nmf * n * sizeof(float));
-    c->aux.ensure((nT + 6) * n * sizeof(float));
+    c->aux.ensure((nT + 6) * n * sizeof(float));  // T (1 or 3), 6 shear fluxes
     float* F = c->fields.as<float>();
     float* Mf = c->model.as<float>();
     HIP_OK(hipMemsetAsync(F, 0, 9 * n * sizeof(float), c->stream));
@@ -384,12 +392,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
     inject(prev, 0, 1);
 
-    constexpr int TX = 4;
-    EArgs Af = A;  // flux pass: the whole z range
-    const int nblk_f = tile_grid(Af, g, TX, 0, g.nz);
-    const int nblk = tile_grid(A, g, TX, 6, g.nz - 6);  // z slabs: elastic_zslab
-    const int nslab = 12 * g.nx * g.ny;
-    const int nshell = shell_count(1, g.nx, g.ny, g.nz);  // surface points (traction SATs)
+    const int TX = c->tune_tx;
+    const int nblk = tile_grid(c, A, g, TX);
+    const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
@@ -420,18 +425,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        Af.curr = curr; Af.prev = prev; Af.next = next;
-        Af.prev_tapered = A.prev_tapered;
         if (vti) {
-            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, true>), dim3(nblk_f), dim3(64, TX), 0, c->stream, Af); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, true>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 3, [&] { hipLaunchKernelGGL(elastic_zslab<true>, dim3((nslab + 255) / 256, 3), dim3(256), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_faces<true>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            if (TX == 8) step_elastic<8, true>(c, A, nblk, nshell);
+            else step_elastic<4, true>(c, A, nblk, nshell);
         } else {
-            timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, false>), dim3(nblk_f), dim3(64, TX), 0, c->stream, Af); });
-            timed(c, 0, [&] { hipLaunchKernelGGL((elastic_step<TX, false>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-            timed(c, 3, [&] { hipLaunchKernelGGL(elastic_zslab<false>, dim3((nslab + 255) / 256, 3), dim3(256), 0, c->stream, A); });
-            timed(c, 2, [&] { hipLaunchKernelGGL(elastic_faces<false>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+            if (TX == 8) step_elastic<8, false>(c, A, nblk, nshell);
+            else step_elastic<4, false>(c, A, nblk, nshell);
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
@@ -628,6 +627,21 @@ int fwi3d_propagate(fwi3d_ctx* c, int physics, int adj, const float* model, cons
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
+    if (!c) return FWI3D_EINVAL;
+    if (param == FWI3D_TUNE_YCHUNK) {
+        if (value < 0) return fail(c, Err{FWI3D_EINVAL, "ychunk must be >= 0 (0: default)"});
+        c->tune_ychunk = value;
+        return FWI3D_OK;
+    }
+    if (param == FWI3D_TUNE_TX) {
+        if (value != 4 && value != 8) return fail(c, Err{FWI3D_EINVAL, "tile width TX must be 4 or 8"});
+        c->tune_tx = value;
+        return FWI3D_OK;
+    }
+    return fail(c, Err{FWI3D_EINVAL, "unknown tuning parameter"});
 }
 
 int fwi3d_profile(fwi3d_ctx* c, int enable) {

@@ -98,8 +98,15 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
 /* Kernel timing with HIP events on the context's stream, for roofline accounting
  * (no counterpart in the reference). enable=1 starts collecting (and resets), 0 stops.
  * read() returns the summed milliseconds and launch count since the last enable of
- * kernel class `kernel`: 0 = the time-step kernel (points 6+ cells from the z faces),
- * 1 = the elastic/VTI flux pre-pass, 2 = the face-SAT kernel, 3 = the z-slab kernel. */
+ * kernel class `kernel`: 0 = the interior time-step kernel, 1 = the elastic/VTI flux
+ * pre-pass, 2 = the boundary-shell time-step kernel, 3 = reserved (reads zero). */
+/* Tiling of the y-marching time-step kernels, per context (no counterpart in the
+ * reference; the tile shapes an autotuner picks): FWI3D_TUNE_YCHUNK = y planes per
+ * workgroup (0: default 8), FWI3D_TUNE_TX = x rows per workgroup (4 or 8). Results do
+ * not depend on the tiling. */
+enum fwi3d_tune_param { FWI3D_TUNE_YCHUNK = 0, FWI3D_TUNE_TX = 1 };
+int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
+
 int fwi3d_profile(fwi3d_ctx *ctx, int enable);
 int fwi3d_profile_read(const fwi3d_ctx *ctx, int kernel, double *ms, long long *launches);
 
