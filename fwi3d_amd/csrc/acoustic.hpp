@@ -34,7 +34,9 @@ struct AcuArgs {
     int bc[6];
     float dt, dt2, alpha;  // alpha = 0.2508560249 / free_surface_stiffness
     int prev_tapered;
-    int ychunk, ntz, ntx, nyc;
+    // y-march tiling: tiles of 64 (z) x TX (x) x ychunk (y) from the corner (org, org, org);
+    // the y march of the last chunk stops at yend
+    int ychunk, ntz, ntx, nyc, org, yend;
 };
 
 template <int TX>
@@ -60,10 +62,10 @@ __global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
     const int tx = (L / A.ntz) % A.ntx;
     const int cy = L / (A.ntz * A.ntx);
     const int lz = threadIdx.x, lx = threadIdx.y;
-    const int z0 = tz * TZ, x0 = tx * TX;
+    const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX;
     const int iz = z0 + lz, ix = x0 + lx;
-    const int y0 = cy * A.ychunk;
-    const int y1 = min(ny, y0 + A.ychunk);
+    const int y0 = A.org + cy * A.ychunk;
+    const int y1 = min(A.yend, y0 + A.ychunk);
     // every load uses clamped coordinates: values outside the grid are never used
     // by an interior point, so no load needs a guard
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
@@ -135,7 +137,7 @@ __device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float K
 __global__ __launch_bounds__(256) void acoustic_boundary(AcuArgs A) {
     const Geom g = A.g;
     int ix, iy, iz;
-    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    if (!shell_point(blockIdx.x * 256 + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
     const float* __restrict__ curr = A.curr;
     const float* __restrict__ bb = A.b;
     const float* __restrict__ K = A.K;

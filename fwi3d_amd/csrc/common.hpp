@@ -1,6 +1,6 @@
 // Device-side building blocks shared by the gfx950 kernels: grid geometry,
 // SBP coefficients in constant memory, and per-point operator evaluations used on
-// boundary rows (the interior paths of the fused kernels do not use these).
+// boundary rows (the interior paths of the y-marching kernels do not use these).
 //
 // All kernels of the engine live in one translation unit (engine.hip includes
 // this header once), so the __constant__ tables below are defined exactly once.
@@ -27,9 +27,7 @@ struct Geom {
     __host__ __device__ int at(int ix, int iy, int iz) const { return iy * sy + ix * sx + iz; }
 };
 
-// D2 closure C[row][u-point][b-point], D1 closure rows, S and H weights.
-__constant__ float c_d2b[6][8][8];
-__constant__ float c_d1b[4][6];
+// S and H weights (the closure rows are compile-time constants, below).
 __constant__ float c_sat_s[4];
 __constant__ float c_hnorm[4];
 
@@ -112,38 +110,11 @@ __device__ __forceinline__ float d1_rows(int r, const U& u) {
     }
 }
 
-// Closure rows with the coefficient table in constant memory, all rows in one pass:
-// the form for lanes of one wave sitting on different rows (the z axis), where a
-// per-row branch would run the rows one after another, each waiting for its loads.
-__device__ __forceinline__ float d2_closure_tab(const float* __restrict__ u, const float* __restrict__ b, int base,
-                                                int step, int r) {
-    float bk[8], uk[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        bk[k] = b[base + k * step];
-        uk[k] = u[base + k * step];
-    }
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        float w = 0.f;
-#pragma unroll
-        for (int k = 0; k < 8; k++) w += c_d2b[r][j][k] * bk[k];
-        acc += w * uk[j];
-    }
-    return acc;
-}
-__device__ __forceinline__ float d1_closure_tab(const float* __restrict__ u, int base, int step, int r) {
-    float acc = 0.f;
-#pragma unroll
-    for (int j = 0; j < 6; j++) acc += c_d1b[r][j] * u[base + j * step];
-    return acc;
-}
-
 // D2(b) (times h^2) at axis index i (any row) reading global memory: the field and
-// coefficient at point p, axis length n, element stride s. Boundary rows (i < 6 or
-// i >= n-6): along z (s == 1, rows differ across a wave) the one-pass table form,
-// along x / y (one row per wave) the compile-time rows.
+// coefficient at point p, axis length n, element stride s. On boundary rows (i < 6 or
+// i >= n-6) along z (s == 1) the lanes of a wave sit on different rows, so the eight
+// u and b values are loaded once for all lanes and each row's terms then run from
+// registers; along x / y (one row per wave) the row loads only its own operands.
 __device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
                                           int s) {
     if (i >= 6 && i < n - 6) {
@@ -154,7 +125,15 @@ __device__ __forceinline__ float d2_point(const float* __restrict__ u, const flo
     const int r = lo ? i : n - 1 - i;
     const int base = lo ? p - i * s : p + (n - 1 - i) * s;
     const int step = lo ? s : -s;
-    if (s == 1) return d2_closure_tab(u, b, base, step, r);
+    if (s == 1) {
+        float uk[8], bk[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            uk[k] = u[base + k * step];
+            bk[k] = b[base + k * step];
+        }
+        return d2_rows(r, [&](int j) { return uk[j]; }, [&](int k) { return bk[k]; });
+    }
     return d2_rows(r, [&](int j) { return u[base + j * step]; }, [&](int k) { return b[base + k * step]; });
 }
 // D1 (times h) on a boundary row (i < 4 or i >= n-4), reading global memory (row
@@ -164,7 +143,15 @@ __device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, 
     const int r = lo ? i : n - 1 - i;
     const int base = lo ? p - i * s : p + (n - 1 - i) * s;
     const int step = lo ? s : -s;
-    const float v = s == 1 ? d1_closure_tab(u, base, step, r) : d1_rows(r, [&](int j) { return u[base + j * step]; });
+    float v;
+    if (s == 1) {
+        float uk[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) uk[k] = u[base + k * step];
+        v = d1_rows(r, [&](int j) { return uk[j]; });
+    } else {
+        v = d1_rows(r, [&](int j) { return u[base + j * step]; });
+    }
     return lo ? v : -v;
 }
 // D1 (times h) at axis index i reading global memory.

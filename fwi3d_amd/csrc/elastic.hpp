@@ -44,7 +44,9 @@ struct EArgs {
     int bc[6];
     float dt, dt2;
     int prev_tapered;
-    int ychunk, ntz, ntx, nyc;
+    // y-march tiling: tiles of 64 (z) x TX (x) x ychunk (y) from the corner (org, org, org);
+    // the y march of the last chunk stops at yend
+    int ychunk, ntz, ntx, nyc, org, yend;
 };
 
 // ------------------------------------------------------------------------------------
@@ -65,9 +67,9 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
     const int lz = threadIdx.x, lx = threadIdx.y;
-    const int z0 = tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
+    const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
     const bool valid = (iz < nz) && (ix < nx);
-    const int y0 = cy * A.ychunk, y1 = min(ny, y0 + A.ychunk);
+    const int y0 = A.org + cy * A.ychunk, y1 = min(A.yend, y0 + A.ychunk);
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
     auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
@@ -191,8 +193,8 @@ __global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
     const int lz = threadIdx.x, lx = threadIdx.y;
-    const int z0 = tz * TZ, x0 = tx * TX, iz = z0 + lz, ix = x0 + lx;
-    const int y0 = cy * A.ychunk, y1 = min(ny, y0 + A.ychunk);
+    const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
+    const int y0 = A.org + cy * A.ychunk, y1 = min(A.yend, y0 + A.ychunk);
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
     const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
@@ -371,17 +373,15 @@ __device__ __forceinline__ float e_scale(const Geom& g, const int* bc, const ECo
     return any ? 1.f + sc : 1.f;
 }
 
-template <bool VTI>
-__global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
-    const Geom g = A.g;
-    int ix, iy, iz;
-    if (!shell_point(blockIdx.x * blockDim.x + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
-    const int c = blockIdx.y;  // displacement component of this thread
+// One shell point, displacement component c (a compile-time constant, so every axis
+// and field choice below folds away and nothing is indexed at run time).
+template <bool VTI, int c>
+__device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, int iy, int iz) {
+    const Geom& g = A.g;
     constexpr int NT = VTI ? 3 : 1;
     const int n3 = g.n;
     const float* __restrict__ curr = A.curr;
     const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-    // selects instead of small arrays: c is a runtime value and indexed arrays would spill
     auto BF = [&](int k) -> const float* { return k == 0 ? A.mu : (k == 1 ? (VTI ? A.c66 : A.mu) : (VTI ? A.c11b : A.mu)); };
     auto IDX = [&](int a) { return a == 0 ? ix : (a == 1 ? iy : iz); };
     const float* F = A.aux + NT * n3;
@@ -444,6 +444,18 @@ __global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
         if (face) nv = nv / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, c, A.dt);
         A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
     }
+}
+
+// Grid: (shell points / 256, 3 components). Workgroups go to the XCDs in dispatch
+// order, so every XCD gets the same mix of the cheap y/x slabs and the costlier z
+// slabs (a contiguous range per XCD would leave the z-slab XCDs finishing last).
+template <bool VTI>
+__global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
+    int ix, iy, iz;
+    if (!shell_point(blockIdx.x * 256 + threadIdx.x, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
+    if (blockIdx.y == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);
+    else if (blockIdx.y == 1) elastic_boundary_point<VTI, 1>(A, ix, iy, iz);
+    else elastic_boundary_point<VTI, 2>(A, ix, iy, iz);
 }
 
 // Source injection, all three displacement components (blockIdx.y):

@@ -27,15 +27,8 @@ bool g_tables_loaded[64] = {false};
 
 void load_tables(int dev) {
     if (dev >= 0 && dev < 64 && g_tables_loaded[dev]) return;
-    float d2[6][8][8] = {};
-    for (const auto& e : kD2Closure) d2[e.r][e.j][e.k] = (float)((double)e.num / (double)e.den);
-    float d1[4][6];
-    for (int r = 0; r < 4; r++)
-        for (int j = 0; j < 6; j++) d1[r][j] = (float)kD1Closure[r][j];
     float s[4], h[4];
     for (int k = 0; k < 4; k++) { s[k] = (float)kSatS[k]; h[k] = (float)kHnorm[k]; }
-    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_d2b), d2, sizeof(d2)));
-    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_d1b), d1, sizeof(d1)));
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sat_s), s, sizeof(s)));
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_hnorm), h, sizeof(h)));
     if (dev >= 0 && dev < 64) g_tables_loaded[dev] = true;
@@ -179,12 +172,18 @@ Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
 // `ychunk` planes (context tuning; default 8: many short chunks balance better than
 // fewer long ones, measured on MI355X at 200^3).
 template <class Args>
-int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX) {
-    A.ntz = (g.nz + 63) / 64;
-    A.ntx = (g.nx + TX - 1) / TX;
-    A.ychunk = std::max(4, std::min(g.ny, c->tune_ychunk > 0 ? c->tune_ychunk : 8));
-    A.nyc = (g.ny + A.ychunk - 1) / A.ychunk;
-    return A.ntz * A.ntx * A.nyc;
+int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX, bool interior) {
+    // interior: the points 6+ cells inside every face, [6, n-6) on each axis; otherwise
+    // the whole grid (the flux pass)
+    const int org = interior ? 6 : 0;
+    const int ez = g.nz - 2 * org, ex = g.nx - 2 * org, ey = g.ny - 2 * org;
+    A.org = org;
+    A.yend = g.ny - org;
+    A.ntz = (ez + 63) / 64;
+    A.ntx = (ex + TX - 1) / TX;
+    A.ychunk = std::max(4, std::min(std::max(ey, 1), c->tune_ychunk > 0 ? c->tune_ychunk : 8));
+    A.nyc = (ey + A.ychunk - 1) / A.ychunk;
+    return A.ntz * A.ntx * A.nyc;  // 0 when the interior is empty (an axis of 12 points)
 }
 
 // ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero
@@ -275,7 +274,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
     A.dt = dt; A.dt2 = dt * dt;
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
-    const int nblk = tile_grid(c, A, g, TX);
+    const int nblk = tile_grid(c, A, g, TX, true);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
 
     const int nthr = 256;
@@ -308,7 +307,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        timed(c, 0, [&] {
+        if (nblk > 0) timed(c, 0, [&] {
             if (TX == 8) hipLaunchKernelGGL(acoustic_interior<8>, dim3(nblk), dim3(64, 8), 0, c->stream, A);
             else hipLaunchKernelGGL(acoustic_interior<4>, dim3(nblk), dim3(64, 4), 0, c->stream, A);
         });
@@ -333,10 +332,13 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 }
 
 // One elastic/VTI time step: flux pre-pass, interior y-march, boundary shell.
+// A carries the flux pass's tiling (whole grid, nflux workgroups), AI the interior's
+// (nint workgroups, 0 when the interior is empty).
 template <int TX, bool VTI>
-void step_elastic(fwi3d_ctx* c, const EArgs& A, int nblk, int nshell) {
-    timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, VTI>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
-    timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, VTI>), dim3(nblk), dim3(64, TX), 0, c->stream, A); });
+void step_elastic(fwi3d_ctx* c, const EArgs& A, int nflux, const EArgs& AI, int nint, int nshell) {
+    timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, A); });
+    if (nint > 0)
+        timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, VTI>), dim3(nint), dim3(64, TX), 0, c->stream, AI); });
     timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<VTI>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
 }
 
@@ -393,7 +395,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     inject(prev, 0, 1);
 
     const int TX = c->tune_tx;
-    const int nblk = tile_grid(c, A, g, TX);
+    const int nblk = tile_grid(c, A, g, TX, false);
+    EArgs AI = A;
+    const int nint = tile_grid(c, AI, g, TX, true);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
@@ -425,12 +429,14 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
+        AI.curr = curr; AI.prev = prev; AI.next = next;
+        AI.prev_tapered = A.prev_tapered;
         if (vti) {
-            if (TX == 8) step_elastic<8, true>(c, A, nblk, nshell);
-            else step_elastic<4, true>(c, A, nblk, nshell);
+            if (TX == 8) step_elastic<8, true>(c, A, nblk, AI, nint, nshell);
+            else step_elastic<4, true>(c, A, nblk, AI, nint, nshell);
         } else {
-            if (TX == 8) step_elastic<8, false>(c, A, nblk, nshell);
-            else step_elastic<4, false>(c, A, nblk, nshell);
+            if (TX == 8) step_elastic<8, false>(c, A, nblk, AI, nint, nshell);
+            else step_elastic<4, false>(c, A, nblk, AI, nint, nshell);
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
