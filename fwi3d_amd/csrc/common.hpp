@@ -154,6 +154,17 @@ __device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, 
     }
     return lo ? v : -v;
 }
+// D1 (times h) of closure row r from values already staged on chip: a(k) returns the
+// k-th point counted from the face (k < 6); lo as for d1_closure. Same arithmetic as
+// d1_closure.
+template <class A_>
+__device__ __forceinline__ float d1_closure_at(bool lo, int r, const A_& a) {
+    float uk[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) uk[k] = a(k);
+    const float v = d1_rows(r, [&](int j) { return uk[j]; });
+    return lo ? v : -v;
+}
 // D1 (times h) at axis index i reading global memory.
 __device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, int i, int n, int s) {
     if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);

@@ -51,8 +51,10 @@ struct EArgs {
 
 // ------------------------------------------------------------------------------------
 // Pass 1: stresses and fluxes over the whole grid.
+// Occupancy: 5 waves per SIMD (96 VGPRs, 8 B/lane spilled) measured faster than the
+// compiler's 4 waves without spills.
 template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) void elastic_flux(EArgs A) {
     constexpr int TZ = 64;
     __shared__ float su[3][TX + 4][TZ + 4];
     const float* __restrict__ curr = A.curr;
@@ -64,8 +66,11 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
     const float* __restrict__ c66f = A.c66;
     const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
     const float dx = A.g.dx, dy = A.g.dy, dz = A.g.dz;
+    // y chunks fastest in the workgroup order: the chunks above and below, which re-read
+    // this chunk's edge planes, run on the same XCD at the same time (measured faster
+    // for this kernel; the interior kernel prefers z tiles fastest)
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
+    const int cy = L % A.nyc, tz = (L / A.nyc) % A.ntz, tx = L / (A.nyc * A.ntz);
     const int lz = threadIdx.x, lx = threadIdx.y;
     const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
     const bool valid = (iz < nz) && (ix < nx);
@@ -76,6 +81,16 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
     HaloMap<TX> H;
     H.init(lx, lz, x0, z0, nx, nz, sx);
     const bool inx = ix >= 4 && ix < nx - 4, inz = iz >= 4 && iz < nz - 4;
+    // D1 closure rows along z and x: the row's six points counted from the face sit in
+    // the staged plane (LDS column / row `zb|xb + k * zs|xs`) whenever the tile and its
+    // 2-cell halo cover them; otherwise (a tile ending within 3 points of a face) the
+    // row reads global memory. x is uniform per wave, z differs across lanes 0-3 / the
+    // last 4.
+    const bool zlo = iz < 4, xlo = ix < 4;
+    const int zb = zlo ? 2 - z0 : nz + 1 - z0, zs = zlo ? 1 : -1, zr = zlo ? iz : nz - 1 - iz;
+    const int xb = xlo ? 2 - x0 : nx + 1 - x0, xs = xlo ? 1 : -1, xr = xlo ? ix : nx - 1 - ix;
+    const bool zlds = zlo ? (z0 <= 2) : (zb - 5 >= 0 && zb < TZ + 4);
+    const bool xlds = xlo ? (x0 <= 2 && xb + 5 < TX + 4) : (xb - 5 >= 0 && xb < TX + 4);
     float q[3][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -110,12 +125,14 @@ __global__ __launch_bounds__(64 * TX) void elastic_flux(EArgs A) {
             float dxu[3], dyu[3], dzu[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                dxu[c] = inx ? D1C1 * (su[c][lx + 3][lz + 2] - su[c][lx + 1][lz + 2]) + D1C2 * (su[c][lx + 4][lz + 2] - su[c][lx][lz + 2])
-                             : d1_closure(curr + c * n3, p, ix, nx, sx);
+                dxu[c] = inx    ? D1C1 * (su[c][lx + 3][lz + 2] - su[c][lx + 1][lz + 2]) + D1C2 * (su[c][lx + 4][lz + 2] - su[c][lx][lz + 2])
+                         : xlds ? d1_closure_at(xlo, xr, [&](int k) { return su[c][xb + k * xs][lz + 2]; })
+                                : d1_closure(curr + c * n3, p, ix, nx, sx);
                 dyu[c] = iny ? D1C1 * (q[c][3] - q[c][1]) + D1C2 * (q[c][4] - q[c][0])
                              : d1_closure(curr + c * n3, p, iy, ny, sy);
-                dzu[c] = inz ? D1C1 * (su[c][lx + 2][lz + 3] - su[c][lx + 2][lz + 1]) + D1C2 * (su[c][lx + 2][lz + 4] - su[c][lx + 2][lz])
-                             : d1_closure(curr + c * n3, p, iz, nz, 1);
+                dzu[c] = inz    ? D1C1 * (su[c][lx + 2][lz + 3] - su[c][lx + 2][lz + 1]) + D1C2 * (su[c][lx + 2][lz + 4] - su[c][lx + 2][lz])
+                         : zlds ? d1_closure_at(zlo, zr, [&](int k) { return su[c][lx + 2][zb + k * zs]; })
+                                : d1_closure(curr + c * n3, p, iz, nz, 1);
                 dxu[c] = dxu[c] / dx;
                 dyu[c] = dyu[c] / dy;
                 dzu[c] = dzu[c] / dz;
@@ -168,8 +185,10 @@ __device__ __forceinline__ float d2_factor(int c, int a) {
 // LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_zx F_xy F_zy F_xz.
 // Register queues (y neighbours): u (3), b-fields, T_y, F_yx, F_yz.
 // b-fields: elastic {mu}; VTI {mu, c66, c11b}.
+// Occupancy: 4 waves per SIMD (<= 128 VGPRs, no spills) measured faster than the
+// compiler's 3 waves (142-157 VGPRs); 5 waves spill heavily and are much slower.
 template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX) void elastic_interior(EArgs A) {
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) void elastic_interior(EArgs A) {
     constexpr int TZ = 64;
     constexpr int NB = VTI ? 3 : 1;   // D2 coefficient fields
     constexpr int NT = VTI ? 3 : 1;   // normal-stress fields
@@ -449,8 +468,9 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
 // Grid: (shell points / 256, 3 components). Workgroups go to the XCDs in dispatch
 // order, so every XCD gets the same mix of the cheap y/x slabs and the costlier z
 // slabs (a contiguous range per XCD would leave the z-slab XCDs finishing last).
+// Occupancy: 6 waves per SIMD (80 VGPRs, 20-36 B/lane spilled) measured faster than 5.
 template <bool VTI>
-__global__ __launch_bounds__(256) void elastic_boundary(EArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void elastic_boundary(EArgs A) {
     int ix, iy, iz;
     if (!shell_point(blockIdx.x * 256 + threadIdx.x, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
     if (blockIdx.y == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);

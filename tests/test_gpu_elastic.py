@@ -95,6 +95,20 @@ def test_elastic_forward_matches_oracle(eng, physics, case, kind):
 
 
 @pytest.mark.parametrize("physics", ["elastic", "vti"])
+def test_elastic_tile_edges_near_faces(eng, physics):
+    """nz = 66 and nx = 27: the last z tile (z 64-65) and the last x tiles end within 3
+    points of the high faces, so those D1 closure rows of the flux pass read global
+    memory while the rows of the other tiles read the staged plane; both against the
+    oracle."""
+    print(f"[{physics} tile edges] shape=(20, 27, 66)", flush=True)
+    got, want, _, _ = run_pair(eng, physics, 1, CASES["mixed"], nt=120, shape=(20, 27, 66))
+    e = rel(got, want)
+    print(f"  rel L2 vs oracle f32 {e:.2e}")
+    assert np.isfinite(got).all() and np.abs(want).max() > 0
+    assert e <= TRACE_TOL
+
+
+@pytest.mark.parametrize("physics", ["elastic", "vti"])
 def test_elastic_forward_fp64(eng, physics):
     """Error against the FP64 oracle stays at the level of the FP32 oracle's."""
     cfg = CASES["free+absorb+taper"]
