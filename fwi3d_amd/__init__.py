@@ -27,6 +27,8 @@ EXPORTS = (
     "fwi3d_op_param_defaults", "fwi3d_analyze_model", "fwi3d_apply_forward", "fwi3d_apply_jacobianT",
     "fwi3d_apply_forward_device", "fwi3d_apply_jacobianT_device", "fwi3d_fwi_shot_device",
     "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run", "fwi3d_tune",
+    "fwi3d_batch_create", "fwi3d_batch_destroy", "fwi3d_batch_footprint", "fwi3d_batch_add", "fwi3d_batch_step",
+    "fwi3d_batch_query", "fwi3d_batch_wait", "fwi3d_batch_status",
 )
 
 
@@ -98,6 +100,17 @@ def lib():
         L.fwi3d_lbfgs_param_defaults.argtypes = [C.POINTER(LbfgsParam)]
         L.fwi3d_lbfgs_run.argtypes = [C.c_void_p, C.POINTER(LbfgsParam), C.c_longlong, C.c_void_p, C.c_void_p,
                                       C.c_void_p, OBJECTIVE, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.fwi3d_batch_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.fwi3d_batch_destroy.argtypes = [C.c_void_p]
+        L.fwi3d_batch_footprint.restype = C.c_longlong
+        L.fwi3d_batch_footprint.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.fwi3d_batch_add.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                      C.c_float, C.POINTER(C.c_int)]
+        L.fwi3d_batch_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.fwi3d_batch_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.fwi3d_batch_wait.argtypes = [C.c_void_p]
+        L.fwi3d_batch_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -161,12 +174,16 @@ class Engine:
 
     def __init__(self, device=0):
         self._h = C.c_void_p()
+        self._batches = []
         rc = lib().fwi3d_create(C.byref(self._h), int(device))
         if rc != 0:
             raise RuntimeError(f"fwi3d_create failed ({rc})")
 
     def close(self):
         if self._h:
+            for b in self._batches:  # the context destroys its batches with it
+                b._h = C.c_void_p()
+            self._batches = []
             lib().fwi3d_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -231,6 +248,79 @@ class Engine:
         n = C.c_longlong()
         self._check(lib().fwi3d_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)), "fwi3d_profile_read")
         return ms.value, n.value
+
+
+class Batch:
+    """Batched forward modelling (fwi3d_batch_*): shots of any physics and grid join
+    between time steps and advance together, one launch per kernel class and step.
+    Pointers are device integers (torch.Tensor.data_ptr()); the caller keeps the
+    model, source and receiver buffers alive until the shot's query() reports done."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        self._h = C.c_void_p()
+        engine._check(lib().fwi3d_batch_create(engine._h, C.byref(self._h)), "fwi3d_batch_create")
+        engine._batches.append(self)
+
+    def close(self):
+        if self._h:
+            lib().fwi3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+            if self in self.engine._batches:
+                self.engine._batches.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def footprint(physics, shape, inj, ext):
+        """Device bytes one shot holds while it is in the batch."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        I, E = inj.cstruct(), ext.cstruct()
+        return int(lib().fwi3d_batch_footprint(ph, nx, ny, nz, C.byref(I), C.byref(E)))
+
+    def add(self, physics, model_ptr, src_ptr, rcv_ptr, inj, ext, par, shape, d):
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        I, E = inj.cstruct(), ext.cstruct()
+        sid = C.c_int(-1)
+        self.engine._check(lib().fwi3d_batch_add(self._h, ph, C.c_void_p(model_ptr), C.c_void_p(src_ptr),
+                                                 C.c_void_p(rcv_ptr) if rcv_ptr else None, C.byref(I), C.byref(E),
+                                                 C.byref(par), nx, ny, nz, C.c_float(d[0]), C.c_float(d[1]),
+                                                 C.c_float(d[2]), C.byref(sid)), "fwi3d_batch_add")
+        return sid.value
+
+    def step(self, nsteps=1):
+        """Advance every shot in the batch by up to nsteps time steps; returns the
+        number of shots still in the batch."""
+        n = C.c_int(0)
+        self.engine._check(lib().fwi3d_batch_step(self._h, int(nsteps), C.byref(n)), "fwi3d_batch_step")
+        return n.value
+
+    def run(self):
+        while self.step(1 << 20) > 0:
+            pass
+        self.wait()
+
+    def query(self, sid):
+        """1 when the shot's traces are complete, 0 while it runs, -1 if unknown."""
+        st = C.c_int(-2)
+        self.engine._check(lib().fwi3d_batch_query(self._h, int(sid), C.byref(st)), "fwi3d_batch_query")
+        return st.value
+
+    def wait(self):
+        self.engine._check(lib().fwi3d_batch_wait(self._h), "fwi3d_batch_wait")
+
+    def status(self):
+        """(shots in the batch, device bytes they hold, global step)."""
+        n, b, it = C.c_int(), C.c_longlong(), C.c_int()
+        self.engine._check(lib().fwi3d_batch_status(self._h, C.byref(n), C.byref(b), C.byref(it)),
+                           "fwi3d_batch_status")
+        return n.value, b.value, it.value
 
 
 def op_param(**kw):

@@ -39,8 +39,10 @@ struct AcuArgs {
     int ychunk, ntz, ntx, nyc, org, yend;
 };
 
-template <int TX>
-__global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
+// One workgroup of the interior y-march: tile tile_id() of the tiling in A (the
+// single-shot kernel and the batched kernel share this body).
+template <int TX, class LFn>
+__device__ __forceinline__ void acoustic_interior_tile(const AcuArgs A, const LFn& tile_id) {
     constexpr int TZ = 64;
     __shared__ float su[TX + 4][TZ + 4];
     __shared__ float sb[TX + 4][TZ + 4];
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(64 * TX) void acoustic_interior(AcuArgs A) {
     const float dt2 = A.dt2;
     const bool ptap = A.prev_tapered != 0;
 
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = tile_id();
     const int tz = L % A.ntz;
     const int tx = (L / A.ntz) % A.ntx;
     const int cy = L / (A.ntz * A.ntx);
@@ -134,10 +136,10 @@ __device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float K
 
 // One boundary-shell point: closures, SATs (faces in the reference's order top,
 // bottom, left, right, front, back), update, scaling, taper.
-__global__ __launch_bounds__(256) void acoustic_boundary(AcuArgs A) {
-    const Geom g = A.g;
+__device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b) {
+    const Geom& g = A.g;
     int ix, iy, iz;
-    if (!shell_point(blockIdx.x * 256 + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    if (!shell_point(b * 256 + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
     const float* __restrict__ curr = A.curr;
     const float* __restrict__ bb = A.b;
     const float* __restrict__ K = A.K;
@@ -211,11 +213,10 @@ __device__ __forceinline__ float inj_value(const InjTab& I, const float* __restr
 
 // Acoustic: next[p] += taper( dt^2 * K * inj / scale ) at the 27 points of each source.
 // mode 0: time-step injection into next; mode 1: initial prev = 0.5 K dt^2 inj(src[0]).
-__global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
-                                int mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= I.npts * 27) return;
-    const Geom g = A.g;
+// Thread t of the injection (t < 27 * npts).
+__device__ __forceinline__ void acoustic_inject_one(float* __restrict__ out, const AcuArgs& A, const InjTab& I,
+                                                    const float* __restrict__ src, int nt, int it, int mode, int t) {
+    const Geom& g = A.g;
     const float* __restrict__ K = A.K;
     const float dt = A.dt;
     const int r = t / 27, q = t % 27;
@@ -233,13 +234,16 @@ __global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, co
     }
     atomicAdd(out + p, add);
 }
+__global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
+                                int mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < I.npts * 27) acoustic_inject_one(out, A, I, src, nt, it, mode, t);
+}
 
 // rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract);
 // component comp = blockIdx.y reads field u + comp * g.n.
-__global__ void extract_points(const float* __restrict__ u3, Geom g, InjTab I, float* __restrict__ rcv, int nt, int it) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= I.npts) return;
-    const int comp = blockIdx.y;
+__device__ __forceinline__ void extract_one(const float* __restrict__ u3, const Geom& g, const InjTab& I,
+                                            float* __restrict__ rcv, int nt, int it, int r, int comp) {
     const float* __restrict__ u = u3 + comp * g.n;
     const float* xw = I.xw + r * 2 * I.nw + I.nw;
     const float* yw = I.yw + r * 2 * I.nw + I.nw;
@@ -250,6 +254,10 @@ __global__ void extract_points(const float* __restrict__ u3, Geom g, InjTab I, f
             for (int c = 0; c < 3; c++)
                 acc += yw[a] * xw[bb] * zw[c] * u[g.at(I.xi[3 * r + bb], I.yi[3 * r + a], I.zi[3 * r + c])];
     rcv[(long long)(comp * I.npts + r) * nt + it] = acc;
+}
+__global__ void extract_points(const float* __restrict__ u3, Geom g, InjTab I, float* __restrict__ rcv, int nt, int it) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < I.npts) extract_one(u3, g, I, rcv, nt, it, r, blockIdx.y);
 }
 
 // Imaging condition of nl_we_op_a::compute_gradients (we_op.cpp:1555-1605), FP64

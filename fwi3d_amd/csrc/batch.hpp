@@ -1,0 +1,356 @@
+// Batched time stepping of many independent shots (forward modelling, uneven grids,
+// any mix of the three physics). The reference models shots one after another: each
+// pass of the shot loop of we_modeling.cpp:84-112 (and of compute_res_and_grad,
+// optimization.cpp:33-190) builds an operator and runs propagate() to completion.
+// Here every admitted shot is a set of workgroups inside the same launches: one time
+// step of the whole batch is a fixed number of launches (flux, interior, shell and
+// injection per physics present, one extraction), whatever the number of shots, and
+// small grids that would leave most of the 256 CUs idle on their own share the chip.
+//
+// Device side (batch_kernels.hpp): a table of per-shot records (BShot) and per kernel
+// class a map from launch block to shot with each shot's first block. The tables
+// change only when shots join or leave. Shots join between steps and leave after their
+// last extraction (continuous batching).
+//
+// Included once, by engine.hip (all kernels share its translation unit).
+#pragma once
+
+// ======================================================================================
+// Host side of the batch.
+
+struct fwi3d_batch {
+    enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VINJ, C_EXT, NCLS };
+    struct Shot {
+        int id = -1;
+        fwi3d::BShot d{};
+        int nblk[NCLS] = {};
+        void* mem = nullptr;
+        size_t bytes = 0;
+    };
+    fwi3d_ctx* c = nullptr;
+    int it = 0;       // global step counter
+    int next_id = 0;
+    int TX = 4;       // x rows per workgroup of every y-march launch of the batch
+    std::vector<Shot> shots;                 // admitted, not yet retired
+    std::vector<std::pair<int, hipEvent_t>> retired;  // id -> event after its last work
+    bool dirty = true;
+    fwi3d::DevBuf tab, maps;
+    int cls_blocks[NCLS] = {};
+    size_t cls_off[NCLS] = {};  // offset (ints) of the class's block->shot map; start array follows it
+    std::vector<int> hmaps;
+    std::vector<fwi3d::BShot> htab;
+    size_t resident = 0;
+};
+
+namespace {
+
+using fwi3d::BShot;
+
+int batch_group(int phys) { return phys == FWI3D_ACOUSTIC ? 0 : (phys == FWI3D_ELASTIC ? 1 : 2); }
+
+// Device bytes of one batched shot: three wavefield sets, model, flux buffer, tapers,
+// injector tables (256-byte aligned parts).
+size_t batch_shot_bytes(int phys, const Geom& g, const fwi3d_injector* inj, const fwi3d_injector* ext) {
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const int ncomp = phys == FWI3D_ACOUSTIC ? 1 : 3;
+    const int nmf = phys == FWI3D_ACOUSTIC ? 2 : (phys == FWI3D_ELASTIC ? 3 : 7);
+    const int naux = phys == FWI3D_ACOUSTIC ? 0 : (phys == FWI3D_ELASTIC ? 7 : 9);
+    const size_t n = (size_t)g.n;
+    return al(3 * ncomp * n * 4) + al(nmf * n * 4) + al(naux * n * 4) + al((size_t)(g.nz + g.nx + g.ny) * 4) +
+           al(inj_bytes(inj)) + al(inj_bytes(ext));
+}
+
+// Rebuild the device tables after shots joined or left.
+void batch_rebuild(fwi3d_batch* b) {
+    using B = fwi3d_batch;
+    HIP_OK(hipStreamSynchronize(b->c->stream));  // the host copies below may still be in flight
+    const int ns = (int)b->shots.size();
+    b->htab.resize(ns);
+    for (int s = 0; s < ns; s++) b->htab[s] = b->shots[s].d;
+    b->hmaps.clear();
+    for (int k = 0; k < B::NCLS; k++) {
+        b->cls_off[k] = b->hmaps.size();
+        int total = 0;
+        std::vector<int> start(ns, 0);
+        for (int s = 0; s < ns; s++) {
+            start[s] = total;
+            for (int q = 0; q < b->shots[s].nblk[k]; q++) b->hmaps.push_back(s);
+            total += b->shots[s].nblk[k];
+        }
+        b->cls_blocks[k] = total;
+        b->hmaps.insert(b->hmaps.end(), start.begin(), start.end());
+    }
+    if (ns > 0) {
+        b->tab.ensure(ns * sizeof(BShot));
+        b->maps.ensure(b->hmaps.size() * sizeof(int) + 256);
+        HIP_OK(hipMemcpyAsync(b->tab.p, b->htab.data(), ns * sizeof(BShot), hipMemcpyHostToDevice, b->c->stream));
+        HIP_OK(hipMemcpyAsync(b->maps.p, b->hmaps.data(), b->hmaps.size() * sizeof(int), hipMemcpyHostToDevice,
+                              b->c->stream));
+    }
+    b->dirty = false;
+}
+
+fwi3d::BMap batch_map(const fwi3d_batch* b, int k) {
+    const int* base = b->maps.as<int>() + b->cls_off[k];
+    return fwi3d::BMap{base, base + b->cls_blocks[k]};
+}
+
+// One global step: extraction, then per physics the time step and the injection.
+void batch_one_step(fwi3d_batch* b) {
+    using B = fwi3d_batch;
+    using namespace fwi3d;
+    hipStream_t st = b->c->stream;
+    const BShot* T = b->tab.as<BShot>();
+    const int it = b->it;
+    auto go = [&](int k, auto kern, int threads, dim3 blk) {
+        if (b->cls_blocks[k] > 0) hipLaunchKernelGGL(kern, dim3(b->cls_blocks[k]), blk, 0, st, T, batch_map(b, k), it);
+        (void)threads;
+    };
+    go(B::C_EXT, extract_batch, 128, dim3(128));
+    if (b->TX == 8) {
+        go(B::C_AINT, acoustic_interior_batch<8>, 0, dim3(64, 8));
+        go(B::C_EFLUX, elastic_flux_batch<8, false>, 0, dim3(64, 8));
+        go(B::C_EINT, elastic_interior_batch<8, false>, 0, dim3(64, 8));
+        go(B::C_VFLUX, elastic_flux_batch<8, true>, 0, dim3(64, 8));
+        go(B::C_VINT, elastic_interior_batch<8, true>, 0, dim3(64, 8));
+    } else {
+        go(B::C_AINT, acoustic_interior_batch<4>, 0, dim3(64, 4));
+        go(B::C_EFLUX, elastic_flux_batch<4, false>, 0, dim3(64, 4));
+        go(B::C_EINT, elastic_interior_batch<4, false>, 0, dim3(64, 4));
+        go(B::C_VFLUX, elastic_flux_batch<4, true>, 0, dim3(64, 4));
+        go(B::C_VINT, elastic_interior_batch<4, true>, 0, dim3(64, 4));
+    }
+    go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
+    go(B::C_ESHELL, elastic_boundary_batch<false>, 0, dim3(256));
+    go(B::C_VSHELL, elastic_boundary_batch<true>, 0, dim3(256));
+    go(B::C_AINJ, acoustic_inject_batch, 0, dim3(256));
+    go(B::C_EINJ, elastic_inject_batch<false>, 0, dim3(256));
+    go(B::C_VINJ, elastic_inject_batch<true>, 0, dim3(256));
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int fwi3d_batch_create(fwi3d_ctx* c, fwi3d_batch** out) {
+    if (!c || !out) return FWI3D_EINVAL;
+    *out = nullptr;
+    fwi3d_batch* b = new fwi3d_batch();
+    b->c = c;
+    b->TX = c->tune_tx;
+    c->batches.push_back(b);
+    *out = b;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_destroy(fwi3d_batch* b) {
+    if (!b) return FWI3D_OK;
+    auto& live = b->c->batches;
+    live.erase(std::remove(live.begin(), live.end(), b), live.end());
+    (void)hipSetDevice(b->c->device);
+    (void)hipStreamSynchronize(b->c->stream);
+    for (auto& s : b->shots)
+        if (s.mem) (void)hipFreeAsync(s.mem, b->c->stream);
+    (void)hipStreamSynchronize(b->c->stream);
+    for (auto& r : b->retired) (void)hipEventDestroy(r.second);
+    b->tab.release();
+    b->maps.release();
+    delete b;
+    return FWI3D_OK;
+}
+
+long long fwi3d_batch_footprint(int physics, int nx, int ny, int nz, const fwi3d_injector* inj,
+                                const fwi3d_injector* ext) {
+    if (!inj || !ext || nx < 12 || ny < 12 || nz < 12) return -1;
+    if (physics != FWI3D_ACOUSTIC && physics != FWI3D_ELASTIC && physics != FWI3D_VTI) return -1;
+    const Geom g = make_geom(nx, ny, nz, 1.f, 1.f, 1.f);
+    return (long long)batch_shot_bytes(physics, g, inj, ext);
+}
+
+int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const float* d_src, float* d_rcv,
+                    const fwi3d_injector* inj, const fwi3d_injector* ext, const fwi3d_param* par, int nx, int ny,
+                    int nz, float dx, float dy, float dz, int* id) {
+    if (!b) return FWI3D_EINVAL;
+    fwi3d_ctx* c = b->c;
+    void* mem = nullptr;
+    try {
+        using namespace fwi3d;
+        validate(physics, par, nx, ny, nz, dx, dy, dz);
+        REQUIRE(par->sub == 0, "batched shots are forward modelling only (sub must be 0: no snapshot store)");
+        REQUIRE(d_model && d_src, "null model or source");
+        check_injector(inj, nx, ny, nz, "source");
+        check_injector(ext, nx, ny, nz, "receiver");
+        REQUIRE(d_rcv || ext->npts == 0, "null receiver buffer");
+        HIP_OK(hipSetDevice(c->device));
+        hipStream_t st = c->stream;
+        const Geom g = make_geom(nx, ny, nz, dx, dy, dz);
+        const size_t bytes = batch_shot_bytes(physics, g, inj, ext);
+        if (hipMallocAsync(&mem, bytes, st) != hipSuccess || !mem)
+            throw Err{FWI3D_ENOMEM, "hipMallocAsync(" + std::to_string(bytes) + " bytes) for a batched shot failed"};
+        auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+        const long long n = g.n;
+        const int ncomp = physics == FWI3D_ACOUSTIC ? 1 : 3;
+        const int npar = physics == FWI3D_ACOUSTIC ? 2 : (physics == FWI3D_ELASTIC ? 3 : 6);
+        const int nmf = physics == FWI3D_ACOUSTIC ? 2 : (physics == FWI3D_ELASTIC ? 3 : 7);
+        const int naux = physics == FWI3D_ACOUSTIC ? 0 : (physics == FWI3D_ELASTIC ? 7 : 9);
+        char* p = (char*)mem;
+        float* F = (float*)p;          p += al(3 * ncomp * n * 4);
+        float* M = (float*)p;          p += al(nmf * n * 4);
+        float* aux = (float*)p;        p += al(naux * n * 4);
+        float* tp = (float*)p;         p += al((size_t)(g.nz + g.nx + g.ny) * 4);
+        char* pinj = p;                p += al(inj_bytes(inj));
+        char* pext = p;
+        HIP_OK(hipMemsetAsync(F, 0, 3 * ncomp * n * 4, st));
+        HIP_OK(hipMemsetAsync(M, 0, nmf * n * 4, st));
+        to_padded_on(st, M, d_model, npar, g);
+        const Tapers T = tapers_to(st, tp, par, g);
+        fwi3d_batch::Shot s;
+        s.id = b->next_id++;
+        s.mem = mem;
+        s.bytes = bytes;
+        BShot& d = s.d;
+        d.phys = physics;
+        d.t0 = b->it;
+        d.nt = par->nt;
+        d.nf = ncomp * n;
+        d.F = F;
+        d.g = g;
+        d.I = upload_inj_to(st, pinj, inj);
+        d.E = upload_inj_to(st, pext, ext);
+        d.src = d_src;
+        d.rcv = d_rcv;
+        const float dt = par->dt;
+        const int yc = c->tune_ychunk;
+        const int TX = b->TX;
+        const int nshell = shell_count(6, g.nx, g.ny, g.nz);
+        d.nshell_blk = (nshell + 255) / 256;
+        d.ninj_blk = (inj->npts * 27 + 255) / 256;
+        d.next_blk = (ext->npts + 127) / 128;
+        using B = fwi3d_batch;
+        const int grp = batch_group(physics);
+        if (physics == FWI3D_ACOUSTIC) {
+            AcuArgs& A = d.a;
+            A.K = M; A.b = M + n; A.tz2 = T.tz2; A.tx2 = T.tx2; A.ty2 = T.ty2; A.g = g;
+            for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
+            A.dt = dt; A.dt2 = dt * dt;
+            A.alpha = 0.2508560249f / par->free_surface_stiffness;
+            s.nblk[B::C_AINT] = tile_grid_yc(A, g, TX, true, yc);
+            s.nblk[B::C_ASHELL] = d.nshell_blk;
+            s.nblk[B::C_AINJ] = d.ninj_blk;
+            A.prev = F; A.curr = F + n; A.next = F + 2 * n;
+            if (inj->npts > 0)
+                hipLaunchKernelGGL(acoustic_inject, dim3(d.ninj_blk), dim3(256), 0, st, F, A, d.I, d_src, d.nt, 0, 1);
+        } else {
+            const bool vti = physics == FWI3D_VTI;
+            EArgs& A = d.ef;
+            A.lam = M; A.mu = M + n; A.rho = M + 2 * n;
+            if (vti) {
+                A.c13 = M + 3 * n; A.eps = M + 4 * n; A.c66 = M + 5 * n; A.c11b = M + 6 * n;
+                hipLaunchKernelGGL(vti_c11b_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, M + 6 * n, A.eps,
+                                   A.mu, n);
+            }
+            A.tz2 = T.tz2; A.tx2 = T.tx2; A.ty2 = T.ty2; A.g = g;
+            for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
+            A.aux = aux;
+            A.dt = dt; A.dt2 = dt * dt;
+            d.ei = A;
+            const int nf = tile_grid_yc(A, g, TX, false, yc);
+            const int ni = tile_grid_yc(d.ei, g, TX, true, yc);
+            s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = nf;
+            s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = ni;
+            s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 3 * d.nshell_blk;
+            s.nblk[grp == 1 ? B::C_EINJ : B::C_VINJ] = 3 * d.ninj_blk;
+            A.prev = F; A.curr = F + 3 * n; A.next = F + 6 * n;
+            if (inj->npts > 0) {
+                if (vti)
+                    hipLaunchKernelGGL(elastic_inject<true>, dim3(d.ninj_blk, 3), dim3(256), 0, st, F, A, d.I, d_src, d.nt, 0, 1);
+                else
+                    hipLaunchKernelGGL(elastic_inject<false>, dim3(d.ninj_blk, 3), dim3(256), 0, st, F, A, d.I, d_src, d.nt, 0, 1);
+            }
+        }
+        s.nblk[B::C_EXT] = ncomp * d.next_blk;
+        HIP_OK(hipGetLastError());
+        mem = nullptr;  // owned by the shot from here
+        b->shots.push_back(s);
+        b->resident += bytes;
+        b->dirty = true;
+        if (id) *id = s.id;
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        if (mem) (void)hipFreeAsync(mem, c->stream);
+        return fail(c, e);
+    }
+}
+
+int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
+    if (!b) return FWI3D_EINVAL;
+    fwi3d_ctx* c = b->c;
+    try {
+        REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+        HIP_OK(hipSetDevice(c->device));
+        for (int q = 0; q < nsteps && !b->shots.empty(); q++) {
+            if (b->dirty) batch_rebuild(b);
+            batch_one_step(b);
+            // retire the shots whose last extraction (local step nt-1) was this step
+            bool left = false;
+            for (size_t s = 0; s < b->shots.size();) {
+                auto& sh = b->shots[s];
+                if (b->it - sh.d.t0 >= sh.d.nt - 1) {
+                    hipEvent_t ev;
+                    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                    HIP_OK(hipEventRecord(ev, c->stream));
+                    HIP_OK(hipFreeAsync(sh.mem, c->stream));
+                    b->resident -= sh.bytes;
+                    b->retired.emplace_back(sh.id, ev);
+                    b->shots.erase(b->shots.begin() + s);
+                    left = true;
+                } else {
+                    s++;
+                }
+            }
+            if (left) b->dirty = true;
+            b->it++;
+        }
+        if (nactive) *nactive = (int)b->shots.size();
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int fwi3d_batch_query(fwi3d_batch* b, int id, int* state) {
+    if (!b || !state) return FWI3D_EINVAL;
+    for (auto& s : b->shots)
+        if (s.id == id) {
+            *state = 0;
+            return FWI3D_OK;
+        }
+    for (auto& r : b->retired)
+        if (r.first == id) {
+            const hipError_t e = hipEventQuery(r.second);
+            if (e == hipSuccess) *state = 1;
+            else if (e == hipErrorNotReady) *state = 0;
+            else return fail(b->c, Err{FWI3D_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(e)});
+            return FWI3D_OK;
+        }
+    *state = -1;
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_wait(fwi3d_batch* b) {
+    if (!b) return FWI3D_EINVAL;
+    const hipError_t e = hipStreamSynchronize(b->c->stream);
+    if (e != hipSuccess) return fail(b->c, Err{FWI3D_EDEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e)});
+    return FWI3D_OK;
+}
+
+int fwi3d_batch_status(const fwi3d_batch* b, int* nshots, long long* resident_bytes, int* step) {
+    if (!b) return FWI3D_EINVAL;
+    if (nshots) *nshots = (int)b->shots.size();
+    if (resident_bytes) *resident_bytes = (long long)b->resident;
+    if (step) *step = b->it;
+    return FWI3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Time-step kernels over a table of shots. Each launch block finds its shot through a
+// block -> shot map and runs the shared tile / point bodies of acoustic.hpp and
+// elastic.hpp with that shot's arguments; the wavefield rotation (prev, curr, next) of
+// a shot follows from the global step `it` and its admission step t0. The batch
+// (batch.hpp) launches them over many shots; the single-shot path launches them with a
+// one-entry table and no map, so a shot computes the same alone, in a batch, or through
+// propagate().
+#pragma once
+#include "acoustic.hpp"
+#include "elastic.hpp"
+
+namespace fwi3d {
+
+struct BShot {
+    int phys;       // FWI3D_ACOUSTIC, FWI3D_ELASTIC or FWI3D_VTI
+    int t0, nt;     // global step of admission, time samples
+    long long nf;   // elements of one wavefield set (components x padded field)
+    float* F;       // three wavefield sets, rotated as prev / curr / next
+    Geom g;
+    AcuArgs a;      // acoustic: interior tiling
+    EArgs ef, ei;   // elastic / VTI: flux-pass tiling, interior tiling
+    InjTab I, E;    // source injector, receiver extractor
+    const float* src;
+    float* rcv;
+    int nshell_blk;  // shell workgroups of one component (256 points each)
+    int ninj_blk;    // injection workgroups of one component (256 threads)
+    int next_blk;    // extraction workgroups of one component (128 receivers)
+};
+
+// Launch block -> shot index, and each shot's first block in the launch. A null map
+// is a launch for the single shot T[0] (the single-shot path uses these kernels too).
+struct BMap {
+    const int* shot;
+    const int* start;
+    __device__ __forceinline__ int shot_of(int L) const { return shot ? shot[L] : 0; }
+    __device__ __forceinline__ int first(int s) const { return start ? start[s] : 0; }
+};
+
+template <class Args>
+__device__ __forceinline__ void bshot_rotate(Args& A, const BShot& S, int k) {
+    A.prev = S.F + (long long)(k % 3) * S.nf;
+    A.curr = S.F + (long long)((k + 1) % 3) * S.nf;
+    A.next = S.F + (long long)((k + 2) % 3) * S.nf;
+    A.prev_tapered = k > 0 ? 1 : 0;
+}
+
+// ---- time-step kernels: block -> (shot, local block); shots outside their stepping
+// range (local step k in [0, nt-2]) exit at once.
+// Occupancy: 8 waves per SIMD (forced, 58 VGPRs) measured faster than the compiler's
+// 6 waves (75 VGPRs).
+template <int TX>
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(8))) void acoustic_interior_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    AcuArgs A = T[s].a;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s);
+    acoustic_interior_tile<TX>(A, [&] { return l; });
+}
+
+__global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    AcuArgs A = T[s].a;
+    bshot_rotate(A, T[s], k);
+    acoustic_boundary_block(A, L - m.first(s));
+}
+
+// Occupancy: 5 waves per SIMD (96 VGPRs, 8 B/lane spilled) measured faster than the
+// compiler's 4 waves without spills.
+template <int TX, bool VTI>
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) void elastic_flux_batch(
+    const BShot* __restrict__ T, BMap m, int it) {
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s);
+    elastic_flux_tile<TX, VTI>(A, [&] { return l; });
+}
+
+// Occupancy: 4 waves per SIMD (<= 128 VGPRs) measured faster than the compiler's 3 waves
+// (142-157 VGPRs); 5 waves spill heavily and are much slower.
+template <int TX, bool VTI>
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) void elastic_interior_batch(
+    const BShot* __restrict__ T, BMap m, int it) {
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ei;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s);
+    elastic_interior_tile<TX, VTI>(A, [&] { return l; });
+}
+
+// Shell workgroups are not XCD-remapped: dispatch order gives every XCD the same mix of
+// the cheap y/x slabs and the costlier z slabs. Occupancy: 6 waves per SIMD (80 VGPRs,
+// 20-40 B/lane spilled) measured faster than 5.
+template <bool VTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void elastic_boundary_batch(
+    const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s), nb = T[s].nshell_blk;
+    elastic_boundary_block<VTI>(A, l % nb, l / nb);
+}
+
+// ---- source injection into `next` after the step (mode 0), receivers before it.
+__global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    AcuArgs A = T[s].a;
+    bshot_rotate(A, T[s], k);
+    const int t = (L - m.first(s)) * 256 + threadIdx.x;
+    if (t < T[s].I.npts * 27) acoustic_inject_one(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t);
+}
+
+template <bool VTI>
+__global__ void elastic_inject_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s), nb = T[s].ninj_blk;
+    const int t = (l % nb) * 256 + threadIdx.x;
+    if (t < T[s].I.npts * 27) elastic_inject_one<VTI>(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t, l / nb);
+}
+
+// Receivers of local step k in [0, nt-1] read the current field.
+__global__ void extract_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 1) return;
+    const int l = L - m.first(s), nb = T[s].next_blk;
+    const int r = (l % nb) * 128 + threadIdx.x;
+    if (r >= T[s].E.npts) return;
+    const float* curr = T[s].F + (long long)((k + 1) % 3) * T[s].nf;
+    extract_one(curr, T[s].g, T[s].E, T[s].rcv, T[s].nt, k, r, l / nb);
+}
+
+}  // namespace fwi3d

@@ -51,8 +51,15 @@ struct DevBuf {
 
 }  // namespace fwi3d
 
+struct fwi3d_batch;
+
 struct fwi3d_ctx {
     int device = 0;
+    std::vector<fwi3d_batch*> batches;  // live batches on this context (destroyed with it)
+    // one-shot table of the single-shot time step (a BShot record, engine.hip): host
+    // bytes kept until the next propagate call, and its device copy
+    std::vector<char> shot1_host;
+    fwi3d::DevBuf shot1_dev;
     hipStream_t stream = nullptr;
     std::string err;
     fwi3d::DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;

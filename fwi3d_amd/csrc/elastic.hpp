@@ -51,10 +51,9 @@ struct EArgs {
 
 // ------------------------------------------------------------------------------------
 // Pass 1: stresses and fluxes over the whole grid.
-// Occupancy: 5 waves per SIMD (96 VGPRs, 8 B/lane spilled) measured faster than the
-// compiler's 4 waves without spills.
-template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) void elastic_flux(EArgs A) {
+// Launched by elastic_flux_batch (batch_kernels.hpp) at 5 waves per SIMD.
+template <int TX, bool VTI, class LFn>
+__device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile_id) {
     constexpr int TZ = 64;
     __shared__ float su[3][TX + 4][TZ + 4];
     const float* __restrict__ curr = A.curr;
@@ -69,7 +68,7 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) vo
     // y chunks fastest in the workgroup order: the chunks above and below, which re-read
     // this chunk's edge planes, run on the same XCD at the same time (measured faster
     // for this kernel; the interior kernel prefers z tiles fastest)
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = tile_id();
     const int cy = L % A.nyc, tz = (L / A.nyc) % A.ntz, tx = L / (A.nyc * A.ntz);
     const int lz = threadIdx.x, lx = threadIdx.y;
     const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
@@ -185,10 +184,9 @@ __device__ __forceinline__ float d2_factor(int c, int a) {
 // LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_zx F_xy F_zy F_xz.
 // Register queues (y neighbours): u (3), b-fields, T_y, F_yx, F_yz.
 // b-fields: elastic {mu}; VTI {mu, c66, c11b}.
-// Occupancy: 4 waves per SIMD (<= 128 VGPRs, no spills) measured faster than the
-// compiler's 3 waves (142-157 VGPRs); 5 waves spill heavily and are much slower.
-template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) void elastic_interior(EArgs A) {
+// Launched by elastic_interior_batch (batch_kernels.hpp) at 4 waves per SIMD.
+template <int TX, bool VTI, class LFn>
+__device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ A, const LFn& tile_id) {
     constexpr int TZ = 64;
     constexpr int NB = VTI ? 3 : 1;   // D2 coefficient fields
     constexpr int NT = VTI ? 3 : 1;   // normal-stress fields
@@ -209,7 +207,7 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) vo
     const bool ptap = A.prev_tapered != 0;
     const float* bf[3] = {A.mu, VTI ? A.c66 : A.mu, VTI ? A.c11b : A.mu};
     const float* Ff = A.aux + NT * n3;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = tile_id();
     const int tz = L % A.ntz, tx = (L / A.ntz) % A.ntx, cy = L / (A.ntz * A.ntx);
     const int lz = threadIdx.x, lx = threadIdx.y;
     const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
@@ -465,29 +463,23 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
     }
 }
 
-// Grid: (shell points / 256, 3 components). Workgroups go to the XCDs in dispatch
-// order, so every XCD gets the same mix of the cheap y/x slabs and the costlier z
-// slabs (a contiguous range per XCD would leave the z-slab XCDs finishing last).
-// Occupancy: 6 waves per SIMD (80 VGPRs, 20-36 B/lane spilled) measured faster than 5.
+// Shell workgroup b (256 points of the shell enumeration), displacement component c.
 template <bool VTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void elastic_boundary(EArgs A) {
+__device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, int c) {
     int ix, iy, iz;
-    if (!shell_point(blockIdx.x * 256 + threadIdx.x, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
-    if (blockIdx.y == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);
-    else if (blockIdx.y == 1) elastic_boundary_point<VTI, 1>(A, ix, iy, iz);
+    if (!shell_point(b * 256 + threadIdx.x, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
+    if (c == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);
+    else if (c == 1) elastic_boundary_point<VTI, 1>(A, ix, iy, iz);
     else elastic_boundary_point<VTI, 2>(A, ix, iy, iz);
 }
 
 // Source injection, all three displacement components (blockIdx.y):
 // next += taper(dt^2 inj / rho / scale); mode 1: initial prev = 0.5 dt^2 inj(src[0]) / rho.
 template <bool VTI>
-__global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
-                               int mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= I.npts * 27) return;
-    const int comp = blockIdx.y;
+__device__ __forceinline__ void elastic_inject_one(float* __restrict__ out3, const EArgs& A, const InjTab& I,
+                                                   const float* __restrict__ src, int nt, int it, int mode, int t, int comp) {
     float* __restrict__ out = out3 + comp * A.g.n;
-    const Geom g = A.g;
+    const Geom& g = A.g;
     const float dt = A.dt;
     const int r = t / 27, qq = t % 27;
     const int a = qq / 9, bb = (qq / 3) % 3, c = qq % 3;
@@ -504,6 +496,12 @@ __global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, cons
         add = taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
     }
     atomicAdd(out + p, add);
+}
+template <bool VTI>
+__global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
+                               int mode) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < I.npts * 27) elastic_inject_one<VTI>(out3, A, I, src, nt, it, mode, t, blockIdx.y);
 }
 
 // Imaging condition of nl_we_op_e / nl_we_op_vti ::compute_gradients

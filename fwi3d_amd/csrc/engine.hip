@@ -16,6 +16,7 @@
 #include "context.hpp"
 #include "common.hpp"
 #include "elastic.hpp"
+#include "batch_kernels.hpp"
 #include "sbp_tables.hpp"
 #include "snapshots.hpp"
 
@@ -67,23 +68,27 @@ void taper_table(std::vector<float>& t, int n, int lo, int hi, float strength) {
 }
 
 // Copy a model/field set [k][ny][nx][nz] (unpadded, device) into the padded layout.
-void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g) {
+void to_padded_on(hipStream_t stream, float* dst, const float* src, int k, const Geom& g) {
     for (int q = 0; q < k; q++)
         HIP_OK(hipMemcpy2DAsync(dst + q * g.n, g.pz * sizeof(float), src + (size_t)q * g.nx * g.ny * g.nz,
                                 g.nz * sizeof(float), g.nz * sizeof(float), (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,
-                                c->stream));
+                                stream));
+}
+void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g) { to_padded_on(c->stream, dst, src, k, g); }
+
+// Device bytes of one injector's tables (ints then floats).
+size_t inj_bytes(const fwi3d_injector* h) {
+    const size_t nw = h->kind ? 6 : 3;
+    return 3 * (size_t)3 * h->npts * sizeof(int) + 3 * (2 * nw * h->npts) * sizeof(float) + 256;
 }
 
-// Upload injector tables into `buf`: ints then floats.
-InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
+// Upload injector tables to device memory at `base` (inj_bytes(h) bytes).
+InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h) {
     InjTab t{};
     t.npts = h->npts;
     t.kind = h->kind;
     t.nw = h->kind ? 6 : 3;
     const size_t ni = (size_t)3 * h->npts, nw = (size_t)2 * t.nw * h->npts;
-    const size_t bytes = 3 * ni * sizeof(int) + 3 * nw * sizeof(float) + 256;
-    buf.ensure(bytes);
-    char* base = buf.as<char>();
     int* xi = (int*)base;
     int* yi = xi + ni;
     int* zi = yi + ni;
@@ -91,15 +96,21 @@ InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
     float* yw = xw + nw;
     float* zw = yw + nw;
     if (h->npts > 0) {
-        HIP_OK(hipMemcpyAsync(xi, h->xind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(yi, h->yind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(zi, h->zind, ni * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(xw, h->xw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(yw, h->yw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(zw, h->zw, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(xi, h->xind, ni * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(yi, h->yind, ni * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(zi, h->zind, ni * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(xw, h->xw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(yw, h->yw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(zw, h->zw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
     }
     t.xi = xi; t.yi = yi; t.zi = zi; t.xw = xw; t.yw = yw; t.zw = zw;
     return t;
+}
+
+// Upload injector tables into the context buffer `buf`.
+InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
+    buf.ensure(inj_bytes(h));
+    return upload_inj_to(c->stream, buf.as<char>(), h);
 }
 
 void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char* what) {
@@ -148,31 +159,43 @@ void drain_events(fwi3d_ctx* c) {
     c->ev_kind.clear();
 }
 
+// Upload a one-shot table for the time-step kernels of the single-shot path.
+const BShot* upload_shot1(fwi3d_ctx* c, const BShot& d) {
+    c->shot1_host.resize(sizeof(BShot));
+    std::memcpy(c->shot1_host.data(), &d, sizeof(BShot));
+    c->shot1_dev.ensure(sizeof(BShot));
+    HIP_OK(hipMemcpyAsync(c->shot1_dev.p, c->shot1_host.data(), sizeof(BShot), hipMemcpyHostToDevice, c->stream));
+    return c->shot1_dev.as<BShot>();
+}
+
 // --------------------------------------------------------------------------------------
 struct Tapers {
     const float *tz2, *tx2, *ty2;
 };
 
-Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
+// Taper tables (nz + nx + ny floats) uploaded to `dtz`.
+Tapers tapers_to(hipStream_t stream, float* dtz, const fwi3d_param* par, const Geom& g) {
     std::vector<float> tz, tx, ty;
     taper_table(tz, g.nz, par->taper[0], par->taper[1], par->taper_strength);
     taper_table(tx, g.nx, par->taper[2], par->taper[3], par->taper_strength);
     taper_table(ty, g.ny, par->taper[4], par->taper[5], par->taper_strength);
-    c->tapers.ensure((tz.size() + tx.size() + ty.size()) * sizeof(float));
-    float* dtz = c->tapers.as<float>();
     float* dtx = dtz + tz.size();
     float* dty = dtx + tx.size();
-    HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(dtz, tz.data(), tz.size() * 4, hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(dtx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemcpyAsync(dty, ty.data(), ty.size() * 4, hipMemcpyHostToDevice, stream));
     return Tapers{dtz, dtx, dty};
+}
+Tapers setup_tapers(fwi3d_ctx* c, const fwi3d_param* par, const Geom& g) {
+    c->tapers.ensure((size_t)(g.nz + g.nx + g.ny) * sizeof(float));
+    return tapers_to(c->stream, c->tapers.as<float>(), par, g);
 }
 
 // Tiling of the y-marching kernels: 64(z) x TX(x) column tiles, y in chunks of
 // `ychunk` planes (context tuning; default 8: many short chunks balance better than
 // fewer long ones, measured on MI355X at 200^3).
 template <class Args>
-int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX, bool interior) {
+int tile_grid_yc(Args& A, const Geom& g, int TX, bool interior, int ychunk) {
     // interior: the points 6+ cells inside every face, [6, n-6) on each axis; otherwise
     // the whole grid (the flux pass)
     const int org = interior ? 6 : 0;
@@ -181,9 +204,13 @@ int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX, bool interior)
     A.yend = g.ny - org;
     A.ntz = (ez + 63) / 64;
     A.ntx = (ex + TX - 1) / TX;
-    A.ychunk = std::max(4, std::min(std::max(ey, 1), c->tune_ychunk > 0 ? c->tune_ychunk : 8));
+    A.ychunk = std::max(4, std::min(std::max(ey, 1), ychunk > 0 ? ychunk : 8));
     A.nyc = (ey + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;  // 0 when the interior is empty (an axis of 12 points)
+}
+template <class Args>
+int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX, bool interior) {
+    return tile_grid_yc(A, g, TX, interior, c->tune_ychunk);
 }
 
 // ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero
@@ -276,6 +303,11 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
     const int nblk = tile_grid(c, A, g, TX, true);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
+    // one-shot table for the time-step kernels (rotation from the step index)
+    BShot T1h{};
+    T1h.phys = FWI3D_ACOUSTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = n; T1h.F = F; T1h.g = g; T1h.a = A;
+    T1h.nshell_blk = (nshell + 255) / 256;
+    const BShot* T1 = upload_shot1(c, T1h);
 
     const int nthr = 256;
     const int nbi = (I.npts * 27 + nthr - 1) / nthr;
@@ -308,10 +340,12 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
         if (nblk > 0) timed(c, 0, [&] {
-            if (TX == 8) hipLaunchKernelGGL(acoustic_interior<8>, dim3(nblk), dim3(64, 8), 0, c->stream, A);
-            else hipLaunchKernelGGL(acoustic_interior<4>, dim3(nblk), dim3(64, 4), 0, c->stream, A);
+            if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, it);
+            else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, it);
         });
-        timed(c, 2, [&] { hipLaunchKernelGGL(acoustic_boundary, dim3((nshell + 255) / 256), dim3(256), 0, c->stream, A); });
+        timed(c, 2, [&] {
+            hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, it);
+        });
         if (I.npts > 0)
             hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, next, A, I, d_src, nt, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
@@ -332,14 +366,20 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 }
 
 // One elastic/VTI time step: flux pre-pass, interior y-march, boundary shell.
-// A carries the flux pass's tiling (whole grid, nflux workgroups), AI the interior's
-// (nint workgroups, 0 when the interior is empty).
+// The one-shot table T carries the flux pass's tiling (whole grid, nflux workgroups)
+// and the interior's (nint workgroups, 0 when the interior is empty).
 template <int TX, bool VTI>
-void step_elastic(fwi3d_ctx* c, const EArgs& A, int nflux, const EArgs& AI, int nint, int nshell) {
-    timed(c, 1, [&] { hipLaunchKernelGGL((elastic_flux<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, A); });
+void step_elastic(fwi3d_ctx* c, const BShot* T, int it, int nflux, int nint, int nshell_blk) {
+    timed(c, 1, [&] {
+        hipLaunchKernelGGL((elastic_flux_batch<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, T, BMap{}, it);
+    });
     if (nint > 0)
-        timed(c, 0, [&] { hipLaunchKernelGGL((elastic_interior<TX, VTI>), dim3(nint), dim3(64, TX), 0, c->stream, AI); });
-    timed(c, 2, [&] { hipLaunchKernelGGL(elastic_boundary<VTI>, dim3((nshell + 255) / 256, 3), dim3(256), 0, c->stream, A); });
+        timed(c, 0, [&] {
+            hipLaunchKernelGGL((elastic_interior_batch<TX, VTI>), dim3(nint), dim3(64, TX), 0, c->stream, T, BMap{}, it);
+        });
+    timed(c, 2, [&] {
+        hipLaunchKernelGGL(elastic_boundary_batch<VTI>, dim3(3 * nshell_blk), dim3(256), 0, c->stream, T, BMap{}, it);
+    });
 }
 
 void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, const float* d_src, float* d_rcv,
@@ -399,6 +439,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     EArgs AI = A;
     const int nint = tile_grid(c, AI, g, TX, true);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
+    // one-shot table for the time-step kernels (rotation from the step index)
+    BShot T1h{};
+    T1h.phys = vti ? FWI3D_VTI : FWI3D_ELASTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = 3 * n; T1h.F = F; T1h.g = g;
+    T1h.ef = A; T1h.ei = AI;
+    T1h.nshell_blk = (nshell + 255) / 256;
+    const BShot* T1 = upload_shot1(c, T1h);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
@@ -429,14 +475,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        AI.curr = curr; AI.prev = prev; AI.next = next;
-        AI.prev_tapered = A.prev_tapered;
         if (vti) {
-            if (TX == 8) step_elastic<8, true>(c, A, nblk, AI, nint, nshell);
-            else step_elastic<4, true>(c, A, nblk, AI, nint, nshell);
+            if (TX == 8) step_elastic<8, true>(c, T1, it, nblk, nint, T1h.nshell_blk);
+            else step_elastic<4, true>(c, T1, it, nblk, nint, T1h.nshell_blk);
         } else {
-            if (TX == 8) step_elastic<8, false>(c, A, nblk, AI, nint, nshell);
-            else step_elastic<4, false>(c, A, nblk, AI, nint, nshell);
+            if (TX == 8) step_elastic<8, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
+            else step_elastic<4, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
         }
         inject(next, it, 0);
         float* t = prev; prev = curr; curr = next; next = t;
@@ -475,6 +519,8 @@ int fail(fwi3d_ctx* c, const Err& e) {
 
 }  // namespace
 
+#include "batch.hpp"
+
 // ======================================================================================
 extern "C" {
 
@@ -501,13 +547,14 @@ int fwi3d_create(fwi3d_ctx** out, int device) {
 
 int fwi3d_destroy(fwi3d_ctx* c) {
     if (!c) return FWI3D_OK;
+    while (!c->batches.empty()) fwi3d_batch_destroy(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
     c->fields.release(); c->model.release(); c->aux.release(); c->snaps.release(); c->snapsc.release();
     c->tapers.release(); c->injtab.release(); c->exttab.release(); c->tmpio.release();
     c->opmodel.release(); c->opscal.release(); c->optrace.release(); c->opgrad.release(); c->opsyn.release();
-    c->opt.release();
+    c->opt.release(); c->shot1_dev.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FWI3D_OK;

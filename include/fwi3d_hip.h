@@ -221,6 +221,37 @@ void fwi3d_lbfgs_param_defaults(fwi3d_lbfgs_param *p);
 int fwi3d_lbfgs_run(fwi3d_ctx *ctx, const fwi3d_lbfgs_param *par, long long n, float *d_model, float *d_grad,
                     const float *d_h0, fwi3d_objective fn, void *user, double *f_history, int *counters);
 
+/* ---------------------------------------------------------------------------------
+ * Batched forward modelling. Replaces the one-shot-at-a-time loops that call
+ * apply_forward / propagate per shot (we_modeling.cpp:84-112; the shot loop of
+ * nlls_fwi::compute_res_and_grad, optimization.cpp:33-190): shots of any physics and
+ * grid size join a batch between time steps and advance together, each time step of the
+ * whole batch being one launch per kernel class; a shot leaves after its last sample.
+ * Per shot the arguments mean what they mean for fwi3d_propagate_device with adj = 0
+ * (converted model, source, receiver buffer [ncomp][nrcv][nt]), and par->sub must be 0
+ * (no snapshot store). The caller keeps d_model, d_src and d_rcv alive until
+ * fwi3d_batch_query reports the shot done. Each shot's traces equal those of
+ * fwi3d_propagate_device for the same inputs. Calls on one batch, and on the context it
+ * was created on, must not overlap. */
+typedef struct fwi3d_batch fwi3d_batch;
+int fwi3d_batch_create(fwi3d_ctx *ctx, fwi3d_batch **out);
+int fwi3d_batch_destroy(fwi3d_batch *b);
+/* Device bytes one shot holds while in a batch (for admission by memory footprint);
+ * -1 for invalid arguments. */
+long long fwi3d_batch_footprint(int physics, int nx, int ny, int nz, const fwi3d_injector *inj,
+                                const fwi3d_injector *ext);
+/* Admit a shot; it starts at the batch's next step. *id identifies it in queries. */
+int fwi3d_batch_add(fwi3d_batch *b, int physics, const float *d_model, const float *d_src, float *d_rcv,
+                    const fwi3d_injector *inj, const fwi3d_injector *ext, const fwi3d_param *par, int nx, int ny,
+                    int nz, float dx, float dy, float dz, int *id);
+/* Enqueue up to nsteps time steps of every shot in the batch (asynchronous on the
+ * context's stream); *nactive = shots still in the batch afterwards. */
+int fwi3d_batch_step(fwi3d_batch *b, int nsteps, int *nactive);
+/* *state = 1 when shot id's traces are complete, 0 while it runs, -1 for an unknown id. */
+int fwi3d_batch_query(fwi3d_batch *b, int id, int *state);
+int fwi3d_batch_wait(fwi3d_batch *b);
+int fwi3d_batch_status(const fwi3d_batch *b, int *nshots, long long *resident_bytes, int *step);
+
 #ifdef __cplusplus
 }
 #endif
