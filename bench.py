@@ -186,6 +186,12 @@ def run_batch(args, torch, F, W):
         bufs.append((s, inj, ext, par, m, src, r1, r2))
     torch.cuda.synchronize()
     cells = sum(float(np.prod(s.shape)) * (nt - 1) for s in shots)
+    # tile shapes per grid from the autotuner (used by both paths below)
+    tiles = {}
+    for g in range(3):
+        shp = shots[sum(split[:g])].shape
+        tiles[f"{shp[2]}x{shp[1]}x{shp[0]}"] = eng.autotune("vti", shp, steps=10)
+    print(f"# autotuned (tx, ychunk): {tiles}", file=sys.stderr, flush=True)
 
     # plain path, one shot after another (first shot once untimed, to warm up)
     s, inj, ext, par, m, src, r1, _ = bufs[0]
@@ -208,7 +214,9 @@ def run_batch(args, torch, F, W):
     for s, inj, ext, par, m, src, _, r2 in bufs:
         b.add("vti", m.data_ptr(), src.data_ptr(), r2.data_ptr(), inj, ext, par, s.shape, s.d)
     resident = b.status()[1]
+    b.autotune(window=10)  # online: the first 40 steps time y chunks 4/8/16/32
     b.run()
+    batch_tiles = b.tiles()
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     b.close()
@@ -236,6 +244,9 @@ def run_batch(args, torch, F, W):
         "plain_latency_s": {"p50": round(float(np.percentile(lat_plain, 50)), 4),
                             "p99": round(float(np.percentile(lat_plain, 99)), 4)},
         "batch_resident_gib": round(resident / 2**30, 2),
+        "autotuned_tiles_tx_ychunk": tiles,
+        "batch_online_ychunk": batch_tiles[0],
+        "batch_candidate_ms_per_step": {str(y): round(t, 3) for y, t in zip((4, 8, 16, 32), batch_tiles[1])},
         "traces_identical_to_plain": bool(same),
     }
     print(json.dumps(out), flush=True)

@@ -26,9 +26,9 @@ EXPORTS = (
     "fwi3d_propagate", "fwi3d_propagate_device", "fwi3d_profile", "fwi3d_profile_read",
     "fwi3d_op_param_defaults", "fwi3d_analyze_model", "fwi3d_apply_forward", "fwi3d_apply_jacobianT",
     "fwi3d_apply_forward_device", "fwi3d_apply_jacobianT_device", "fwi3d_fwi_shot_device",
-    "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run", "fwi3d_tune",
+    "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run", "fwi3d_tune", "fwi3d_autotune",
     "fwi3d_batch_create", "fwi3d_batch_destroy", "fwi3d_batch_footprint", "fwi3d_batch_add", "fwi3d_batch_step",
-    "fwi3d_batch_query", "fwi3d_batch_wait", "fwi3d_batch_status",
+    "fwi3d_batch_query", "fwi3d_batch_wait", "fwi3d_batch_status", "fwi3d_batch_autotune", "fwi3d_batch_tiles",
 )
 
 
@@ -111,6 +111,8 @@ def lib():
         L.fwi3d_batch_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.fwi3d_batch_wait.argtypes = [C.c_void_p]
         L.fwi3d_batch_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.fwi3d_batch_autotune.argtypes = [C.c_void_p, C.c_int]
+        L.fwi3d_batch_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -238,6 +240,16 @@ class Engine:
         if tx is not None:
             self._check(lib().fwi3d_tune(self._h, 1, int(tx)), "fwi3d_tune")
 
+    def autotune(self, physics, shape, steps=10):
+        """Measure every tile shape on this grid (fwi3d_autotune); the fastest is used for
+        this physics and grid from now on. Returns (tx, ychunk)."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        tx, yc = C.c_int(), C.c_int()
+        self._check(lib().fwi3d_autotune(self._h, ph, nx, ny, nz, int(steps), C.byref(tx), C.byref(yc)),
+                    "fwi3d_autotune")
+        return tx.value, yc.value
+
     def profile(self, enable=True):
         self._check(lib().fwi3d_profile(self._h, int(bool(enable))), "fwi3d_profile")
 
@@ -314,6 +326,16 @@ class Batch:
 
     def wait(self):
         self.engine._check(lib().fwi3d_batch_wait(self._h), "fwi3d_batch_wait")
+
+    def autotune(self, window=10):
+        """Time y chunks 4, 8, 16, 32 over the next 4 x window steps, then keep the best."""
+        self.engine._check(lib().fwi3d_batch_autotune(self._h, int(window)), "fwi3d_batch_autotune")
+
+    def tiles(self):
+        """(y chunk in force or 0, [ms per step of each candidate])."""
+        yc, ms = C.c_int(), (C.c_float * 4)()
+        self.engine._check(lib().fwi3d_batch_tiles(self._h, C.byref(yc), ms), "fwi3d_batch_tiles")
+        return yc.value, list(ms)
 
     def status(self):
         """(shots in the batch, device bytes they hold, global step)."""

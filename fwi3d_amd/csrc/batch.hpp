@@ -40,6 +40,14 @@ struct fwi3d_batch {
     std::vector<int> hmaps;
     std::vector<fwi3d::BShot> htab;
     size_t resident = 0;
+    int force_yc = 0;  // y chunk of every shot (0: each shot's choice at admission)
+    // online tile tuning: the candidate y chunks each run `tune_window` real steps of the
+    // batch (results do not depend on the tiling), then the fastest is kept
+    static constexpr int NCAND = 4;
+    const int tune_cands[NCAND] = {4, 8, 16, 32};
+    float tune_ms[NCAND] = {0, 0, 0, 0};
+    int tune_window = 0, tune_idx = -1, tune_left = 0;
+    hipEvent_t tev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -88,6 +96,23 @@ void batch_rebuild(fwi3d_batch* b) {
                               b->c->stream));
     }
     b->dirty = false;
+}
+
+// Re-tile every shot of the batch with y chunk yc (block counts change, results do not).
+void batch_retile(fwi3d_batch* b, int yc) {
+    using B = fwi3d_batch;
+    for (auto& sh : b->shots) {
+        BShot& d = sh.d;
+        if (d.phys == FWI3D_ACOUSTIC) {
+            sh.nblk[B::C_AINT] = tile_grid_yc(d.a, d.g, b->TX, true, yc);
+        } else {
+            const bool e = d.phys == FWI3D_ELASTIC;
+            sh.nblk[e ? B::C_EFLUX : B::C_VFLUX] = tile_grid_yc(d.ef, d.g, b->TX, false, yc);
+            sh.nblk[e ? B::C_EINT : B::C_VINT] = tile_grid_yc(d.ei, d.g, b->TX, true, yc);
+        }
+    }
+    b->force_yc = yc;
+    b->dirty = true;
 }
 
 fwi3d::BMap batch_map(const fwi3d_batch* b, int k) {
@@ -154,6 +179,8 @@ int fwi3d_batch_destroy(fwi3d_batch* b) {
         if (s.mem) (void)hipFreeAsync(s.mem, b->c->stream);
     (void)hipStreamSynchronize(b->c->stream);
     for (auto& r : b->retired) (void)hipEventDestroy(r.second);
+    for (auto& e : b->tev)
+        if (e) (void)hipEventDestroy(e);
     b->tab.release();
     b->maps.release();
     delete b;
@@ -221,7 +248,9 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
         d.src = d_src;
         d.rcv = d_rcv;
         const float dt = par->dt;
-        const int yc = c->tune_ychunk;
+        int tx_tuned, yc;  // the batch's launches share one TX; the y chunk is per shot
+        tiles_for(c, physics, g, tx_tuned, yc);
+        if (b->force_yc > 0) yc = b->force_yc;
         const int TX = b->TX;
         const int nshell = shell_count(6, g.nx, g.ny, g.nz);
         d.nshell_blk = (nshell + 255) / 256;
@@ -290,6 +319,12 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
         REQUIRE(nsteps >= 0, "nsteps must be >= 0");
         HIP_OK(hipSetDevice(c->device));
         for (int q = 0; q < nsteps && !b->shots.empty(); q++) {
+            const bool tuning = b->tune_idx >= 0;
+            if (tuning && b->tune_left == b->tune_window) {  // a candidate's window starts
+                batch_retile(b, b->tune_cands[b->tune_idx]);
+                batch_rebuild(b);
+                HIP_OK(hipEventRecord(b->tev[0], c->stream));
+            }
             if (b->dirty) batch_rebuild(b);
             batch_one_step(b);
             // retire the shots whose last extraction (local step nt-1) was this step
@@ -311,12 +346,54 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
             }
             if (left) b->dirty = true;
             b->it++;
+            if (tuning && (--b->tune_left == 0 || b->shots.empty())) {  // the window ends
+                HIP_OK(hipEventRecord(b->tev[1], c->stream));
+                HIP_OK(hipEventSynchronize(b->tev[1]));
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, b->tev[0], b->tev[1]));
+                b->tune_ms[b->tune_idx] = ms / (b->tune_window - b->tune_left);
+                if (++b->tune_idx == fwi3d_batch::NCAND || b->shots.empty()) {
+                    int best = 0;
+                    for (int k = 1; k < b->tune_idx; k++)
+                        if (b->tune_ms[k] < b->tune_ms[best]) best = k;
+                    batch_retile(b, b->tune_cands[best]);
+                    b->tune_idx = -1;
+                } else {
+                    b->tune_left = b->tune_window;
+                }
+            }
         }
         if (nactive) *nactive = (int)b->shots.size();
         return FWI3D_OK;
     } catch (const Err& e) {
         return fail(c, e);
     }
+}
+
+int fwi3d_batch_autotune(fwi3d_batch* b, int window) {
+    if (!b) return FWI3D_EINVAL;
+    fwi3d_ctx* c = b->c;
+    try {
+        REQUIRE(window >= 1, "window must be >= 1 step");
+        HIP_OK(hipSetDevice(c->device));
+        for (auto& e : b->tev)
+            if (!e) HIP_OK(hipEventCreate(&e));
+        b->tune_window = window;
+        b->tune_left = window;
+        b->tune_idx = 0;
+        for (auto& t : b->tune_ms) t = 0;
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int fwi3d_batch_tiles(const fwi3d_batch* b, int* ychunk, float* cand_ms) {
+    if (!b) return FWI3D_EINVAL;
+    if (ychunk) *ychunk = b->force_yc;
+    if (cand_ms)
+        for (int k = 0; k < fwi3d_batch::NCAND; k++) cand_ms[k] = b->tune_ms[k];
+    return FWI3D_OK;
 }
 
 int fwi3d_batch_query(fwi3d_batch* b, int id, int* state) {

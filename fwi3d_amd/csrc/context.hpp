@@ -79,6 +79,11 @@ struct fwi3d_ctx {
     // tiling of the y-marching kernels (fwi3d_tune)
     int tune_ychunk = 0;  // y planes per workgroup (0: default 8)
     int tune_tx = 4;      // x rows per workgroup (4 or 8)
+    // tiles chosen by fwi3d_autotune per physics and grid (they take precedence there)
+    struct Tiles {
+        int physics, nx, ny, nz, tx, ychunk;
+    };
+    std::vector<Tiles> tuned;
     bool prof = false;
     double prof_ms[4] = {0, 0, 0, 0};
     long long prof_n[4] = {0, 0, 0, 0};

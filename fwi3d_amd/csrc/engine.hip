@@ -208,9 +208,16 @@ int tile_grid_yc(Args& A, const Geom& g, int TX, bool interior, int ychunk) {
     A.nyc = (ey + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;  // 0 when the interior is empty (an axis of 12 points)
 }
-template <class Args>
-int tile_grid(const fwi3d_ctx* c, Args& A, const Geom& g, int TX, bool interior) {
-    return tile_grid_yc(A, g, TX, interior, c->tune_ychunk);
+// Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
+// grid on the context, else the context's settings (fwi3d_tune).
+void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk) {
+    TX = c->tune_tx;
+    ychunk = c->tune_ychunk;
+    for (const auto& t : c->tuned)
+        if (t.physics == physics && t.nx == g.nx && t.ny == g.ny && t.nz == g.nz) {
+            TX = t.tx;
+            ychunk = t.ychunk;
+        }
 }
 
 // ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero
@@ -295,13 +302,14 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     float* curr = F + n;
     float* next = F + 2 * n;
 
-    const int TX = c->tune_tx;
+    int TX, ychunk;
+    tiles_for(c, FWI3D_ACOUSTIC, g, TX, ychunk);
     AcuArgs A{};
     A.K = M; A.b = M + n; A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2; A.g = g;
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
     A.dt = dt; A.dt2 = dt * dt;
     A.alpha = 0.2508560249f / par->free_surface_stiffness;
-    const int nblk = tile_grid(c, A, g, TX, true);
+    const int nblk = tile_grid_yc(A, g, TX, true, ychunk);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
     // one-shot table for the time-step kernels (rotation from the step index)
     BShot T1h{};
@@ -434,10 +442,11 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
     inject(prev, 0, 1);
 
-    const int TX = c->tune_tx;
-    const int nblk = tile_grid(c, A, g, TX, false);
+    int TX, ychunk;
+    tiles_for(c, vti ? FWI3D_VTI : FWI3D_ELASTIC, g, TX, ychunk);
+    const int nblk = tile_grid_yc(A, g, TX, false, ychunk);
     EArgs AI = A;
-    const int nint = tile_grid(c, AI, g, TX, true);
+    const int nint = tile_grid_yc(AI, g, TX, true, ychunk);
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
     // one-shot table for the time-step kernels (rotation from the step index)
     BShot T1h{};
@@ -695,6 +704,86 @@ int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
         return FWI3D_OK;
     }
     return fail(c, Err{FWI3D_EINVAL, "unknown tuning parameter"});
+}
+
+int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps, int* tx_out, int* ychunk_out) {
+    if (!c) return FWI3D_EINVAL;
+    const bool prof0 = c->prof;
+    double ms0[4];
+    long long n0[4];
+    std::copy(c->prof_ms, c->prof_ms + 4, ms0);
+    std::copy(c->prof_n, c->prof_n + 4, n0);
+    const int tx0 = c->tune_tx, yc0 = c->tune_ychunk;
+    auto restore = [&] {
+        c->tune_tx = tx0; c->tune_ychunk = yc0; c->prof = prof0;
+        std::copy(ms0, ms0 + 4, c->prof_ms);
+        std::copy(n0, n0 + 4, c->prof_n);
+    };
+    try {
+        REQUIRE(physics == FWI3D_ACOUSTIC || physics == FWI3D_ELASTIC || physics == FWI3D_VTI, "unknown physics");
+        REQUIRE(nx >= 12 && ny >= 12 && nz >= 12, "every grid axis needs at least 12 points (SBP closures)");
+        REQUIRE(steps >= 1 && steps <= 1000, "steps must be 1..1000");
+        HIP_OK(hipSetDevice(c->device));
+        drain_events(c);
+        // a homogeneous synthetic model (vp 3, vs 1.7, rho 2.3, eps 0.1): the kernels'
+        // timing does not depend on the values
+        const size_t N = (size_t)nx * ny * nz;
+        const float vp = 3.f, vs = 1.7f, rho = 2.3f, mu = rho * vs * vs, lam = rho * vp * vp - 2.f * mu;
+        std::vector<float> vals;
+        if (physics == FWI3D_ACOUSTIC) vals = {rho * vp * vp, 1.f / rho};
+        else if (physics == FWI3D_ELASTIC) vals = {lam, mu, rho};
+        else vals = {lam, mu, rho, lam, 0.1f, mu};
+        DevBuf m;
+        m.ensure(vals.size() * N * sizeof(float));
+        std::vector<float> host(N);
+        for (size_t q = 0; q < vals.size(); q++) {
+            std::fill(host.begin(), host.end(), vals[q]);
+            HIP_OK(hipMemcpy(m.as<float>() + q * N, host.data(), N * sizeof(float), hipMemcpyHostToDevice));
+        }
+        fwi3d_injector none{};
+        fwi3d_param par{};
+        par.nt = steps + 1;
+        par.sub = 0;
+        par.dt = 1e-4f;
+        const int bc[6] = {1, 2, 2, 2, 2, 2};
+        for (int f = 0; f < 6; f++) { par.bc[f] = bc[f]; par.taper[f] = 0; }
+        par.taper_strength = 0.05f;
+        par.free_surface_stiffness = 1.05f;
+        par.snap_precision = FWI3D_SNAP_F32;
+        // measure without a previous choice for this grid
+        c->tuned.erase(std::remove_if(c->tuned.begin(), c->tuned.end(),
+                                      [&](const fwi3d_ctx::Tiles& t) {
+                                          return t.physics == physics && t.nx == nx && t.ny == ny && t.nz == nz;
+                                      }),
+                       c->tuned.end());
+        auto run = [&](int tx, int yc) -> double {
+            c->tune_tx = tx;
+            c->tune_ychunk = yc;
+            c->prof = true;
+            for (int k = 0; k < 4; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+            const int rc = fwi3d_propagate_device(c, physics, 0, m.as<float>(), m.as<float>(), nullptr, &none, &none,
+                                                  nullptr, &par, nx, ny, nz, 0.01f, 0.01f, 0.01f);
+            if (rc != FWI3D_OK) throw Err{rc, c->err};
+            return (c->prof_ms[0] + c->prof_ms[1] + c->prof_ms[2]) / steps;
+        };
+        run(4, 8);  // warm-up
+        int best_tx = 4, best_yc = 8;
+        double best = 1e30;
+        for (int tx : {4, 8})
+            for (int yc : {4, 8, 16, 32}) {
+                const double t = run(tx, yc);
+                if (t < best) { best = t; best_tx = tx; best_yc = yc; }
+            }
+        restore();
+        c->tuned.push_back(fwi3d_ctx::Tiles{physics, nx, ny, nz, best_tx, best_yc});
+        if (tx_out) *tx_out = best_tx;
+        if (ychunk_out) *ychunk_out = best_yc;
+        m.release();
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        restore();
+        return fail(c, e);
+    }
 }
 
 int fwi3d_profile(fwi3d_ctx* c, int enable) {

@@ -106,6 +106,12 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
  * not depend on the tiling. */
 enum fwi3d_tune_param { FWI3D_TUNE_YCHUNK = 0, FWI3D_TUNE_TX = 1 };
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
+/* Autotuner (no counterpart in the reference): times `steps` time steps of the given
+ * physics and grid on a synthetic model for every tile shape (TX 4 or 8, y chunk 4, 8,
+ * 16 or 32) with HIP events, keeps the fastest for that physics and grid on the context
+ * (used by propagate and by batch admission in preference to fwi3d_tune's settings) and
+ * returns it. */
+int fwi3d_autotune(fwi3d_ctx *ctx, int physics, int nx, int ny, int nz, int steps, int *tx, int *ychunk);
 
 int fwi3d_profile(fwi3d_ctx *ctx, int enable);
 int fwi3d_profile_read(const fwi3d_ctx *ctx, int kernel, double *ms, long long *launches);
@@ -251,6 +257,13 @@ int fwi3d_batch_step(fwi3d_batch *b, int nsteps, int *nactive);
 int fwi3d_batch_query(fwi3d_batch *b, int id, int *state);
 int fwi3d_batch_wait(fwi3d_batch *b);
 int fwi3d_batch_status(const fwi3d_batch *b, int *nshots, long long *resident_bytes, int *step);
+/* Online tile tuning of the batch (no counterpart in the reference): the next 4 x window
+ * steps run with y chunks 4, 8, 16 and 32 in turn (window steps each, timed with HIP
+ * events), then the fastest is kept for every shot, present and future. Results do not
+ * depend on the tiling. fwi3d_batch_tiles returns the y chunk in force (0: per shot)
+ * and, if cand_ms is not NULL, the measured milliseconds per step of the 4 candidates. */
+int fwi3d_batch_autotune(fwi3d_batch *b, int window);
+int fwi3d_batch_tiles(const fwi3d_batch *b, int *ychunk, float *cand_ms);
 
 #ifdef __cplusplus
 }

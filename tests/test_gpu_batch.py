@@ -63,11 +63,14 @@ def single(eng, s, dev):
     return rcv.cpu().numpy()
 
 
-def run_batch(eng, shots, idx, join_late, dev):
+def run_batch(eng, shots, idx, join_late, dev, tune_window=0):
     """Run shots[idx] in one batch; join_late maps a shot index to the global step at
-    which it joins (others join at step 0). Returns {index: traces}."""
+    which it joins (others join at step 0); tune_window > 0 arms the online tile tuner
+    (the tiling changes during the run). Returns {index: traces}."""
     import torch
     b = F.Batch(eng)
+    if tune_window:
+        b.autotune(tune_window)
     keep, ids, out = [], [], {}
 
     def admit(i):
@@ -96,6 +99,10 @@ def run_batch(eng, shots, idx, join_late, dev):
     b.run()
     assert all(b.query(sid) == 1 for sid in ids)
     assert b.status()[0] == 0 and b.status()[1] == 0
+    if tune_window:
+        yc, ms = b.tiles()
+        print(f"  online tuning: y chunk {yc}, ms/step {['%.3f' % t for t in ms]}", flush=True)
+        assert yc in (4, 8, 16, 32) and all(t > 0 for t in ms)
     b.close()
     return {i: r.cpu().numpy() for i, r in out.items()}
 
@@ -107,7 +114,8 @@ def test_batch_mix_equals_alone_and_single_path():
     eng = F.Engine(0)
     try:
         join_late = {4: 25, 5: 40}  # shot index -> global step at which it joins
-        mixed = run_batch(eng, shots, list(range(len(shots))), join_late, dev)
+        # the mixed run also re-tiles itself during its first 12 steps (online tuning)
+        mixed = run_batch(eng, shots, list(range(len(shots))), join_late, dev, tune_window=3)
         for i in range(len(shots)):
             alone = run_batch(eng, shots, [i], {}, dev)[i]
             want = single(eng, shots[i], dev)
