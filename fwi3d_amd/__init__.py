@@ -29,6 +29,9 @@ EXPORTS = (
     "fwi3d_lbfgs_param_defaults", "fwi3d_lbfgs_run", "fwi3d_tune", "fwi3d_autotune",
     "fwi3d_batch_create", "fwi3d_batch_destroy", "fwi3d_batch_footprint", "fwi3d_batch_add", "fwi3d_batch_step",
     "fwi3d_batch_query", "fwi3d_batch_wait", "fwi3d_batch_status", "fwi3d_batch_autotune", "fwi3d_batch_tiles",
+    "fwi3d_batch_snap_footprint", "fwi3d_batch_add_adjoint", "fwi3d_batch_release", "fwi3d_batch_retired",
+    "fwi3d_server_create", "fwi3d_server_submit", "fwi3d_server_poll", "fwi3d_server_wait", "fwi3d_server_error",
+    "fwi3d_server_result", "fwi3d_server_stats", "fwi3d_server_destroy",
 )
 
 
@@ -75,6 +78,19 @@ class LbfgsParam(C.Structure):
                 ("step_max", C.c_double), ("version", C.c_int)]
 
 
+class RequestStruct(C.Structure):
+    """fwi3d_request (include/fwi3d_hip.h)."""
+    _fields_ = [("kind", C.c_int), ("par", OpParam), ("shot", ShotStruct), ("d_model", C.c_void_p),
+                ("d_src", C.c_void_p), ("d_obs", C.c_void_p), ("d_data", C.c_void_p), ("d_grad64", C.c_void_p)]
+
+
+class ServerCounters(C.Structure):
+    """fwi3d_server_counters (include/fwi3d_hip.h)."""
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("failed", C.c_longlong),
+                ("queued", C.c_int), ("active", C.c_int), ("hbm_in_use", C.c_longlong), ("hbm_peak", C.c_longlong),
+                ("peak_active", C.c_int), ("steps", C.c_longlong)]
+
+
 # int (*)(void *user, const float *d_model, float *d_grad, double *f)
 OBJECTIVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double))
 
@@ -113,6 +129,23 @@ def lib():
         L.fwi3d_batch_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         L.fwi3d_batch_autotune.argtypes = [C.c_void_p, C.c_int]
         L.fwi3d_batch_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.fwi3d_batch_snap_footprint.restype = C.c_longlong
+        L.fwi3d_batch_snap_footprint.argtypes = [C.c_int] * 7
+        L.fwi3d_batch_add_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                              C.c_float, C.c_float, C.POINTER(C.c_int)]
+        L.fwi3d_batch_release.argtypes = [C.c_void_p, C.c_int]
+        L.fwi3d_batch_retired.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.fwi3d_server_create.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]
+        L.fwi3d_server_submit.argtypes = [C.c_void_p, C.POINTER(RequestStruct), C.POINTER(C.c_int)]
+        L.fwi3d_server_poll.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.fwi3d_server_wait.argtypes = [C.c_void_p, C.c_int]
+        L.fwi3d_server_error.restype = C.c_char_p
+        L.fwi3d_server_error.argtypes = [C.c_void_p, C.c_int]
+        L.fwi3d_server_result.argtypes = [C.c_void_p, C.c_int, C.POINTER(OpParam)]
+        L.fwi3d_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerCounters)]
+        L.fwi3d_server_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -306,6 +339,40 @@ class Batch:
                                                  C.c_float(d[2]), C.byref(sid)), "fwi3d_batch_add")
         return sid.value
 
+    @staticmethod
+    def snap_footprint(physics, shape, nt, sub, snap_precision="f32"):
+        """Device bytes of the snapshot store of a forward shot with imaging interval sub."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        sp = SNAP[snap_precision] if isinstance(snap_precision, str) else int(snap_precision)
+        ny, nx, nz = shape
+        return int(lib().fwi3d_batch_snap_footprint(ph, nx, ny, nz, int(nt), int(sub), sp))
+
+    def add_adjoint(self, fwd_id, physics, model_ptr, src_ptr, inj, grad64_ptr, par, shape, d):
+        """Adjoint shot of forward shot fwd_id (which must have left the batch): inject
+        src [ncomp][inj.npts][nt] at the receivers and accumulate the FP64 imaging
+        condition into grad64 from fwd_id's snapshots."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        I = inj.cstruct()
+        sid = C.c_int(-1)
+        self.engine._check(lib().fwi3d_batch_add_adjoint(self._h, int(fwd_id), ph, C.c_void_p(model_ptr),
+                                                         C.c_void_p(src_ptr), C.byref(I), C.c_void_p(grad64_ptr),
+                                                         C.byref(par), nx, ny, nz, C.c_float(d[0]), C.c_float(d[1]),
+                                                         C.c_float(d[2]), C.byref(sid)), "fwi3d_batch_add_adjoint")
+        return sid.value
+
+    def release(self, fwd_id):
+        """Drop the snapshot store of forward shot fwd_id."""
+        self.engine._check(lib().fwi3d_batch_release(self._h, int(fwd_id)), "fwi3d_batch_release")
+
+    def retired(self):
+        """Ids of the shots that left the batch during the last step() call."""
+        n = C.c_int(0)
+        self.engine._check(lib().fwi3d_batch_retired(self._h, None, 0, C.byref(n)), "fwi3d_batch_retired")
+        ids = (C.c_int * max(1, n.value))()
+        self.engine._check(lib().fwi3d_batch_retired(self._h, ids, n.value, C.byref(n)), "fwi3d_batch_retired")
+        return list(ids)[: n.value]
+
     def step(self, nsteps=1):
         """Advance every shot in the batch by up to nsteps time steps; returns the
         number of shots still in the batch."""
@@ -476,3 +543,71 @@ def lbfgs_run(engine, n, model_ptr, grad_ptr, objective, niter, h0_ptr=None, **k
         raise err[0]
     engine._check(rc, "fwi3d_lbfgs_run")
     return list(hist)[: cnt[0] + 1], (cnt[0], cnt[1], cnt[2])
+
+
+REQ_MODEL, REQ_GRADIENT = 0, 1
+
+
+class Server:
+    """Serving loop (fwi3d_server_*): a native server thread admits shot-modelling and
+    shot-gradient requests by HBM footprint into one continuously refilled batch on the
+    engine's device. While it exists the engine belongs to it. Device pointers are
+    integers (torch.Tensor.data_ptr()); the caller keeps the buffers alive until the
+    request is done."""
+
+    def __init__(self, engine, hbm_budget, max_active=64):
+        self.engine = engine
+        self._h = C.c_void_p()
+        self._shots = {}
+        engine._check(lib().fwi3d_server_create(engine._h, int(hbm_budget), int(max_active), C.byref(self._h)),
+                      "fwi3d_server_create")
+
+    def submit(self, kind, par, shot, model_ptr, src_ptr, data_ptr=None, obs_ptr=None, grad64_ptr=None):
+        """kind REQ_MODEL: data <- apply_forward(model); REQ_GRADIENT: one FWI shot
+        (misfit, gradient += into grad64, residual into data if given). Returns an id."""
+        r = RequestStruct(int(kind), par, shot.c, C.c_void_p(model_ptr), C.c_void_p(src_ptr),
+                          C.c_void_p(obs_ptr) if obs_ptr else None, C.c_void_p(data_ptr) if data_ptr else None,
+                          C.c_void_p(grad64_ptr) if grad64_ptr else None)
+        rid = C.c_int(-1)
+        self.engine._check(lib().fwi3d_server_submit(self._h, C.byref(r), C.byref(rid)), "fwi3d_server_submit")
+        self._shots[rid.value] = shot  # position arrays are copied; keep the Shot for symmetry
+        return rid.value
+
+    def poll(self, rid):
+        """(state, misfit, latency_s, queue_s); state 0 queued, 1 running, 2 done, -1 failed."""
+        st, mis, lat, qs = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        self.engine._check(lib().fwi3d_server_poll(self._h, int(rid), C.byref(st), C.byref(mis), C.byref(lat),
+                                                   C.byref(qs)), "fwi3d_server_poll")
+        return st.value, mis.value, lat.value, qs.value
+
+    def wait(self, rid=-1):
+        rc = lib().fwi3d_server_wait(self._h, int(rid))
+        if rc != 0:
+            msg = lib().fwi3d_server_error(self._h, int(rid)) if rid >= 0 else b""
+            raise RuntimeError(f"request {rid} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def error(self, rid):
+        msg = lib().fwi3d_server_error(self._h, int(rid))
+        return msg.decode() if msg else ""
+
+    def result_param(self, rid):
+        p = OpParam()
+        self.engine._check(lib().fwi3d_server_result(self._h, int(rid), C.byref(p)), "fwi3d_server_result")
+        return p
+
+    def stats(self):
+        c = ServerCounters()
+        self.engine._check(lib().fwi3d_server_stats(self._h, C.byref(c)), "fwi3d_server_stats")
+        return {k: getattr(c, k) for k, _ in ServerCounters._fields_}
+
+    def close(self):
+        if self._h:
+            rc = lib().fwi3d_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self.engine._check(rc, "fwi3d_server_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -184,7 +184,11 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b)
 
 // ------------------------------------------------------------------------------------
 // Point sources / receivers. Tables: ind [npts][3], w [npts][2][nw] (row 0 inject,
-// row 1 extract). One thread per (point, 27 neighbours).
+// row 1 extract). Injection runs one thread per distinct grid point touched by the
+// 27-point stencils: tbeg[ntgt+1] / tlst[27 npts] list, per target, the contributions
+// t = 27 r + 9 a + 3 b + c in increasing t, i.e. in the order of the reference's serial
+// loop (point, then y, x, z: injector.cpp:106-115), so overlapping stencils sum in a
+// fixed order and every run gives the same bits.
 struct InjTab {
     const int* xi;
     const int* yi;
@@ -192,7 +196,9 @@ struct InjTab {
     const float* xw;
     const float* yw;
     const float* zw;
-    int npts, kind, nw;
+    const int* tbeg;
+    const int* tlst;
+    int npts, kind, nw, ntgt;
 };
 
 // Injected value of displacement/pressure component `comp` at neighbour (a,b,c) of point r.
@@ -213,31 +219,38 @@ __device__ __forceinline__ float inj_value(const InjTab& I, const float* __restr
 
 // Acoustic: next[p] += taper( dt^2 * K * inj / scale ) at the 27 points of each source.
 // mode 0: time-step injection into next; mode 1: initial prev = 0.5 K dt^2 inj(src[0]).
-// Thread t of the injection (t < 27 * npts).
+// Thread j of the injection handles target j (j < ntgt).
 __device__ __forceinline__ void acoustic_inject_one(float* __restrict__ out, const AcuArgs& A, const InjTab& I,
-                                                    const float* __restrict__ src, int nt, int it, int mode, int t) {
+                                                    const float* __restrict__ src, int nt, int it, int mode, int j) {
     const Geom& g = A.g;
     const float* __restrict__ K = A.K;
     const float dt = A.dt;
-    const int r = t / 27, q = t % 27;
-    const int a = q / 9, bb = (q / 3) % 3, c = q % 3;
-    const int ix = I.xi[3 * r + bb], iy = I.yi[3 * r + a], iz = I.zi[3 * r + c];
+    const int e0 = I.tbeg[j], e1 = I.tbeg[j + 1];
+    int t = I.tlst[e0];
+    const int ix = I.xi[3 * (t / 27) + (t % 27 / 3) % 3], iy = I.yi[3 * (t / 27) + t % 27 / 9],
+              iz = I.zi[3 * (t / 27) + t % 3];
     const int p = g.at(ix, iy, iz);
-    const float v = inj_value(I, src, nt, it, 0, r, a, bb, c);
-    float add;
-    if (mode == 1) {
-        add = v * (0.5f * K[p] * dt * dt);
-    } else {
-        add = dt * dt * v * K[p];
-        add = add / acu_scale(g, A.bc, K[p], A.b[p], ix, iy, iz, dt);
-        add = taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+    float val = out[p];
+    for (int e = e0; e < e1; e++) {
+        t = I.tlst[e];
+        const int r = t / 27, q = t % 27;
+        const float v = inj_value(I, src, nt, it, 0, r, q / 9, (q / 3) % 3, q % 3);
+        float add;
+        if (mode == 1) {
+            add = v * (0.5f * K[p] * dt * dt);
+        } else {
+            add = dt * dt * v * K[p];
+            add = add / acu_scale(g, A.bc, K[p], A.b[p], ix, iy, iz, dt);
+            add = taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+        }
+        val += add;
     }
-    atomicAdd(out + p, add);
+    out[p] = val;
 }
 __global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
                                 int mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < I.npts * 27) acoustic_inject_one(out, A, I, src, nt, it, mode, t);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < I.ntgt) acoustic_inject_one(out, A, I, src, nt, it, mode, j);
 }
 
 // rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract);

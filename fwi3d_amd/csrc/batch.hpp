@@ -12,6 +12,12 @@
 // change only when shots join or leave. Shots join between steps and leave after their
 // last extraction (continuous batching).
 //
+// Gradient shots run in the batch as two shots: a forward shot with a snapshot store
+// (par->sub > 0), whose store outlives it, and, once it has left, an adjoint shot that
+// reads that store and accumulates the imaging condition into an FP64 gradient
+// (fwi3d_batch_add_adjoint). Snapshot writes and imaging events are per-shot launches
+// placed between the batch's time steps exactly where propagate() places them.
+//
 // Included once, by engine.hip (all kernels share its translation unit).
 #pragma once
 
@@ -26,7 +32,27 @@ struct fwi3d_batch {
         int nblk[NCLS] = {};
         void* mem = nullptr;
         size_t bytes = 0;
+        int mode = 0;         // 0 forward, 1 forward with snapshot store, 2 adjoint with gradient
+        int sub = 0;          // imaging interval (modes 1, 2)
+        float dt = 0.f;
+        int store = -1;       // key (forward shot id) of the snapshot store (modes 1, 2)
+        double* grad = nullptr;  // FP64 gradient accumulator (mode 2)
+        std::shared_ptr<std::vector<int>> hinj, hext;  // host injection lists (read by async copies)
     };
+    // Snapshot store of a gradient shot: written by the forward shot, read by the adjoint
+    // shot, freed when the adjoint shot leaves (or on fwi3d_batch_release).
+    struct Store {
+        int key = -1;  // id of the forward shot
+        void* mem = nullptr;
+        size_t bytes = 0;
+        SnapDesc S;
+        int physics = 0, nt = 0, sub = 0, nx = 0, ny = 0, nz = 0;
+        bool complete = false;  // the forward shot has left the batch
+        bool taken = false;     // an adjoint shot reads it
+    };
+    std::vector<Store> stores;
+    size_t store_bytes = 0;
+    std::vector<int> last_retired;  // ids that left during the last fwi3d_batch_step call
     fwi3d_ctx* c = nullptr;
     int it = 0;       // global step counter
     int next_id = 0;
@@ -120,6 +146,43 @@ fwi3d::BMap batch_map(const fwi3d_batch* b, int k) {
     return fwi3d::BMap{base, base + b->cls_blocks[k]};
 }
 
+fwi3d_batch::Store* batch_store(fwi3d_batch* b, int key) {
+    for (auto& st : b->stores)
+        if (st.key == key) return &st;
+    return nullptr;
+}
+
+// Per-shot work of gradient shots at the batch's current step, before its time-step
+// kernels, in propagate()'s order: the forward shot stores slot k/sub when sub divides
+// its local step k (k <= nt-1); the adjoint shot runs the imaging event of snapshot
+// (nt-1-k)/sub when sub divides nt-1-k (0 < k < nt-1) and the event of snapshot 0 at its
+// last step k = nt-1, with the current and the previous adjoint fields.
+void batch_hooks(fwi3d_batch* b) {
+    hipStream_t st = b->c->stream;
+    for (auto& sh : b->shots) {
+        if (sh.mode == 0) continue;
+        const BShot& d = sh.d;
+        const int k = b->it - d.t0, nt = d.nt;
+        if (k < 0 || k > nt - 1) continue;
+        const fwi3d_batch::Store* S = batch_store(b, sh.store);
+        float* curr = d.F + (long long)((k + 1) % 3) * d.nf;
+        const float* prev = d.F + (long long)(k % 3) * d.nf;
+        if (sh.mode == 1) {
+            if (k % sh.sub == 0) snap_put(st, S->S, k / sh.sub, curr);
+            continue;
+        }
+        int itl = -1;
+        if (k == nt - 1) itl = 0;
+        else if (k != 0 && (nt - 1 - k) % sh.sub == 0) itl = (nt - 1 - k) / sh.sub;
+        if (itl < 0) continue;
+        const long long nunp = (long long)d.g.nx * d.g.ny * d.g.nz;
+        if (d.phys == FWI3D_ACOUSTIC)
+            grad_event_acoustic(st, S->S, itl, nt, sh.sub, sh.dt, curr, d.a.K, d.g, sh.grad, sh.grad + nunp);
+        else
+            grad_event_elastic(st, d.phys == FWI3D_VTI, S->S, itl, nt, sh.sub, sh.dt, curr, prev, d.ef, sh.grad, nunp);
+    }
+}
+
 // One global step: extraction, then per physics the time step and the injection.
 void batch_one_step(fwi3d_batch* b) {
     using B = fwi3d_batch;
@@ -177,6 +240,8 @@ int fwi3d_batch_destroy(fwi3d_batch* b) {
     (void)hipStreamSynchronize(b->c->stream);
     for (auto& s : b->shots)
         if (s.mem) (void)hipFreeAsync(s.mem, b->c->stream);
+    for (auto& s : b->stores)
+        if (s.mem) (void)hipFreeAsync(s.mem, b->c->stream);
     (void)hipStreamSynchronize(b->c->stream);
     for (auto& r : b->retired) (void)hipEventDestroy(r.second);
     for (auto& e : b->tev)
@@ -191,30 +256,89 @@ long long fwi3d_batch_footprint(int physics, int nx, int ny, int nz, const fwi3d
                                 const fwi3d_injector* ext) {
     if (!inj || !ext || nx < 12 || ny < 12 || nz < 12) return -1;
     if (physics != FWI3D_ACOUSTIC && physics != FWI3D_ELASTIC && physics != FWI3D_VTI) return -1;
-    const Geom g = make_geom(nx, ny, nz, 1.f, 1.f, 1.f);
-    return (long long)batch_shot_bytes(physics, g, inj, ext);
+    try {
+        const Geom g = make_geom(nx, ny, nz, 1.f, 1.f, 1.f);
+        return (long long)batch_shot_bytes(physics, g, inj, ext);
+    } catch (const Err&) {
+        return -1;
+    }
 }
 
-int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const float* d_src, float* d_rcv,
-                    const fwi3d_injector* inj, const fwi3d_injector* ext, const fwi3d_param* par, int nx, int ny,
-                    int nz, float dx, float dy, float dz, int* id) {
+}  // extern "C"
+
+namespace {
+
+// Snapshot slots of a gradient shot (slot k holds local step k*sub).
+int batch_snap_slots(int nt, int sub) { return 1 + nt / sub; }
+
+size_t batch_store_bytes(int physics, const Geom& g, int nt, int sub, int fmt) {
+    const int slots = batch_snap_slots(nt, sub);
+    const int ncomp = physics == FWI3D_ACOUSTIC ? 1 : 3;
+    return (snap_store_bytes(slots, ncomp, g.n, fmt) + 255) / 256 * 256 + ((size_t)slots * 4 + 255) / 256 * 256;
+}
+
+// Admit a shot. mode 0: forward; 1: forward with a snapshot store (par->sub > 0);
+// 2: adjoint reading the store of forward shot `key`, accumulating into d_grad.
+int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, const float* d_src, float* d_rcv,
+                   const fwi3d_injector* inj, const fwi3d_injector* ext, const fwi3d_param* par, int nx, int ny, int nz,
+                   float dx, float dy, float dz, int key, double* d_grad, int* id) {
     if (!b) return FWI3D_EINVAL;
     fwi3d_ctx* c = b->c;
     void* mem = nullptr;
+    void* smem = nullptr;
     try {
         using namespace fwi3d;
         validate(physics, par, nx, ny, nz, dx, dy, dz);
-        REQUIRE(par->sub == 0, "batched shots are forward modelling only (sub must be 0: no snapshot store)");
         REQUIRE(d_model && d_src, "null model or source");
         check_injector(inj, nx, ny, nz, "source");
         check_injector(ext, nx, ny, nz, "receiver");
         REQUIRE(d_rcv || ext->npts == 0, "null receiver buffer");
+        fwi3d_batch::Store* src_store = nullptr;
+        if (mode == 2) {
+            REQUIRE(d_grad != nullptr, "null gradient accumulator");
+            REQUIRE(par->sub > 0, "an adjoint shot needs sub > 0 (the imaging interval of its forward shot)");
+            src_store = batch_store(b, key);
+            REQUIRE(src_store != nullptr, "no snapshot store of forward shot " + std::to_string(key) + " in this batch");
+            REQUIRE(src_store->complete, "forward shot " + std::to_string(key) + " has not left the batch yet");
+            REQUIRE(!src_store->taken, "the snapshots of forward shot " + std::to_string(key) + " are already in use");
+            REQUIRE(src_store->physics == physics && src_store->nt == par->nt && src_store->sub == par->sub &&
+                        src_store->nx == nx && src_store->ny == ny && src_store->nz == nz &&
+                        src_store->S.fmt == par->snap_precision,
+                    "adjoint shot does not match its forward shot (physics, grid, nt, sub or snap_precision)");
+        }
         HIP_OK(hipSetDevice(c->device));
         hipStream_t st = c->stream;
         const Geom g = make_geom(nx, ny, nz, dx, dy, dz);
         const size_t bytes = batch_shot_bytes(physics, g, inj, ext);
         if (hipMallocAsync(&mem, bytes, st) != hipSuccess || !mem)
             throw Err{FWI3D_ENOMEM, "hipMallocAsync(" + std::to_string(bytes) + " bytes) for a batched shot failed"};
+        fwi3d_batch::Store store;
+        if (mode == 1) {
+            const int ncomp = physics == FWI3D_ACOUSTIC ? 1 : 3;
+            const size_t sb = batch_store_bytes(physics, g, par->nt, par->sub, par->snap_precision);
+            if (hipMallocAsync(&smem, sb, st) != hipSuccess || !smem)
+                throw Err{FWI3D_ENOMEM, "hipMallocAsync(" + std::to_string(sb) + " bytes) for a snapshot store failed"};
+            store.mem = smem;
+            store.bytes = sb;
+            store.S.slots = batch_snap_slots(par->nt, par->sub);
+            store.S.ncomp = ncomp;
+            store.S.n = g.n;
+            store.S.fmt = par->snap_precision;
+            store.S.data = (char*)smem;
+            store.S.scale = (unsigned*)((char*)smem + (snap_store_bytes(store.S.slots, ncomp, g.n, store.S.fmt) + 255) /
+                                                          256 * 256);
+            store.physics = physics; store.nt = par->nt; store.sub = par->sub;
+            store.nx = nx; store.ny = ny; store.nz = nz;
+            // scale words start at zero (max-abs accumulation); slots past the last stored
+            // step, (nt-1)/sub, are never written and read as zero
+            HIP_OK(hipMemsetAsync(store.S.scale, 0, (size_t)store.S.slots * 4, st));
+            const int first_unwritten = (par->nt - 1) / par->sub + 1;
+            if (first_unwritten < store.S.slots) {
+                const size_t se = store.S.slot_elems() * store.S.esz();
+                HIP_OK(hipMemsetAsync(store.S.data + (size_t)first_unwritten * se, 0,
+                                      (size_t)(store.S.slots - first_unwritten) * se, st));
+            }
+        }
         auto al = [](size_t x) { return (x + 255) / 256 * 256; };
         const long long n = g.n;
         const int ncomp = physics == FWI3D_ACOUSTIC ? 1 : 3;
@@ -243,8 +367,10 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
         d.nf = ncomp * n;
         d.F = F;
         d.g = g;
-        d.I = upload_inj_to(st, pinj, inj);
-        d.E = upload_inj_to(st, pext, ext);
+        s.hinj = std::make_shared<std::vector<int>>();
+        s.hext = std::make_shared<std::vector<int>>();
+        d.I = upload_inj_to(st, pinj, inj, *s.hinj);
+        d.E = upload_inj_to(st, pext, ext, *s.hext);
         d.src = d_src;
         d.rcv = d_rcv;
         const float dt = par->dt;
@@ -254,7 +380,7 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
         const int TX = b->TX;
         const int nshell = shell_count(6, g.nx, g.ny, g.nz);
         d.nshell_blk = (nshell + 255) / 256;
-        d.ninj_blk = (inj->npts * 27 + 255) / 256;
+        d.ninj_blk = (d.I.ntgt + 255) / 256;
         d.next_blk = (ext->npts + 127) / 128;
         using B = fwi3d_batch;
         const int grp = batch_group(physics);
@@ -268,7 +394,7 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
             s.nblk[B::C_ASHELL] = d.nshell_blk;
             s.nblk[B::C_AINJ] = d.ninj_blk;
             A.prev = F; A.curr = F + n; A.next = F + 2 * n;
-            if (inj->npts > 0)
+            if (d.I.ntgt > 0)
                 hipLaunchKernelGGL(acoustic_inject, dim3(d.ninj_blk), dim3(256), 0, st, F, A, d.I, d_src, d.nt, 0, 1);
         } else {
             const bool vti = physics == FWI3D_VTI;
@@ -291,7 +417,7 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
             s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 3 * d.nshell_blk;
             s.nblk[grp == 1 ? B::C_EINJ : B::C_VINJ] = 3 * d.ninj_blk;
             A.prev = F; A.curr = F + 3 * n; A.next = F + 6 * n;
-            if (inj->npts > 0) {
+            if (d.I.ntgt > 0) {
                 if (vti)
                     hipLaunchKernelGGL(elastic_inject<true>, dim3(d.ninj_blk, 3), dim3(256), 0, st, F, A, d.I, d_src, d.nt, 0, 1);
                 else
@@ -300,6 +426,20 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
         }
         s.nblk[B::C_EXT] = ncomp * d.next_blk;
         HIP_OK(hipGetLastError());
+        s.mode = mode;
+        s.sub = par->sub;
+        s.dt = dt;
+        s.grad = d_grad;
+        if (mode == 1) {
+            store.key = s.id;
+            s.store = s.id;
+            b->stores.push_back(store);
+            b->store_bytes += store.bytes;
+            smem = nullptr;  // owned by the store from here
+        } else if (mode == 2) {
+            s.store = key;
+            src_store->taken = true;
+        }
         mem = nullptr;  // owned by the shot from here
         b->shots.push_back(s);
         b->resident += bytes;
@@ -308,7 +448,63 @@ int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const flo
         return FWI3D_OK;
     } catch (const Err& e) {
         if (mem) (void)hipFreeAsync(mem, c->stream);
+        if (smem) (void)hipFreeAsync(smem, c->stream);
         return fail(c, e);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fwi3d_batch_add(fwi3d_batch* b, int physics, const float* d_model, const float* d_src, float* d_rcv,
+                    const fwi3d_injector* inj, const fwi3d_injector* ext, const fwi3d_param* par, int nx, int ny,
+                    int nz, float dx, float dy, float dz, int* id) {
+    if (!par) return b ? fail(b->c, Err{FWI3D_EINVAL, "null parameters"}) : FWI3D_EINVAL;
+    return batch_add_impl(b, par->sub > 0 ? 1 : 0, physics, d_model, d_src, d_rcv, inj, ext, par, nx, ny, nz, dx, dy,
+                          dz, -1, nullptr, id);
+}
+
+int fwi3d_batch_add_adjoint(fwi3d_batch* b, int fwd_id, int physics, const float* d_model, const float* d_src,
+                            const fwi3d_injector* inj, double* d_grad64, const fwi3d_param* par, int nx, int ny, int nz,
+                            float dx, float dy, float dz, int* id) {
+    fwi3d_injector none{};
+    return batch_add_impl(b, 2, physics, d_model, d_src, nullptr, inj, &none, par, nx, ny, nz, dx, dy, dz, fwd_id,
+                          d_grad64, id);
+}
+
+int fwi3d_batch_release(fwi3d_batch* b, int fwd_id) {
+    if (!b) return FWI3D_EINVAL;
+    auto& v = b->stores;
+    for (size_t q = 0; q < v.size(); q++)
+        if (v[q].key == fwd_id) {
+            if (!v[q].complete || v[q].taken)
+                return fail(b->c, Err{FWI3D_ESTATE, "the snapshots of shot " + std::to_string(fwd_id) +
+                                                        " are still being written or read"});
+            const hipError_t e = hipFreeAsync(v[q].mem, b->c->stream);
+            b->store_bytes -= v[q].bytes;
+            v.erase(v.begin() + q);
+            if (e != hipSuccess) return fail(b->c, Err{FWI3D_EDEVICE, std::string("hipFreeAsync: ") + hipGetErrorString(e)});
+            return FWI3D_OK;
+        }
+    return fail(b->c, Err{FWI3D_EINVAL, "no snapshot store of shot " + std::to_string(fwd_id)});
+}
+
+int fwi3d_batch_retired(const fwi3d_batch* b, int* ids, int cap, int* n) {
+    if (!b || !n || cap < 0 || (cap > 0 && !ids)) return FWI3D_EINVAL;
+    *n = (int)b->last_retired.size();
+    for (int k = 0; k < *n && k < cap; k++) ids[k] = b->last_retired[k];
+    return FWI3D_OK;
+}
+
+long long fwi3d_batch_snap_footprint(int physics, int nx, int ny, int nz, int nt, int sub, int snap_precision) {
+    if (nx < 12 || ny < 12 || nz < 12 || nt < 2 || sub < 1 || snap_precision < 0 || snap_precision > 2) return -1;
+    if (physics != FWI3D_ACOUSTIC && physics != FWI3D_ELASTIC && physics != FWI3D_VTI) return -1;
+    try {
+        const Geom g = make_geom(nx, ny, nz, 1.f, 1.f, 1.f);
+        return (long long)batch_store_bytes(physics, g, nt, sub, snap_precision);
+    } catch (const Err&) {
+        return -1;
     }
 }
 
@@ -318,6 +514,7 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
     try {
         REQUIRE(nsteps >= 0, "nsteps must be >= 0");
         HIP_OK(hipSetDevice(c->device));
+        b->last_retired.clear();
         for (int q = 0; q < nsteps && !b->shots.empty(); q++) {
             const bool tuning = b->tune_idx >= 0;
             if (tuning && b->tune_left == b->tune_window) {  // a candidate's window starts
@@ -326,6 +523,7 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
                 HIP_OK(hipEventRecord(b->tev[0], c->stream));
             }
             if (b->dirty) batch_rebuild(b);
+            batch_hooks(b);
             batch_one_step(b);
             // retire the shots whose last extraction (local step nt-1) was this step
             bool left = false;
@@ -337,7 +535,20 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
                     HIP_OK(hipEventRecord(ev, c->stream));
                     HIP_OK(hipFreeAsync(sh.mem, c->stream));
                     b->resident -= sh.bytes;
+                    if (sh.mode == 1) {
+                        batch_store(b, sh.store)->complete = true;
+                    } else if (sh.mode == 2) {  // the store's last reader has left
+                        auto& v = b->stores;
+                        for (size_t q = 0; q < v.size(); q++)
+                            if (v[q].key == sh.store) {
+                                HIP_OK(hipFreeAsync(v[q].mem, c->stream));
+                                b->store_bytes -= v[q].bytes;
+                                v.erase(v.begin() + q);
+                                break;
+                            }
+                    }
                     b->retired.emplace_back(sh.id, ev);
+                    b->last_retired.push_back(sh.id);
                     b->shots.erase(b->shots.begin() + s);
                     left = true;
                 } else {
@@ -425,7 +636,7 @@ int fwi3d_batch_wait(fwi3d_batch* b) {
 int fwi3d_batch_status(const fwi3d_batch* b, int* nshots, long long* resident_bytes, int* step) {
     if (!b) return FWI3D_EINVAL;
     if (nshots) *nshots = (int)b->shots.size();
-    if (resident_bytes) *resident_bytes = (long long)b->resident;
+    if (resident_bytes) *resident_bytes = (long long)(b->resident + b->store_bytes);
     if (step) *step = b->it;
     return FWI3D_OK;
 }

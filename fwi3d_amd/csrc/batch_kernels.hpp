@@ -23,7 +23,7 @@ struct BShot {
     const float* src;
     float* rcv;
     int nshell_blk;  // shell workgroups of one component (256 points each)
-    int ninj_blk;    // injection workgroups of one component (256 threads)
+    int ninj_blk;    // injection workgroups of one component (256 targets each)
     int next_blk;    // extraction workgroups of one component (128 receivers)
 };
 
@@ -125,7 +125,7 @@ __global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, int i
     AcuArgs A = T[s].a;
     bshot_rotate(A, T[s], k);
     const int t = (L - m.first(s)) * 256 + threadIdx.x;
-    if (t < T[s].I.npts * 27) acoustic_inject_one(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t);
+    if (t < T[s].I.ntgt) acoustic_inject_one(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t);
 }
 
 template <bool VTI>
@@ -138,7 +138,7 @@ __global__ void elastic_inject_batch(const BShot* __restrict__ T, BMap m, int it
     bshot_rotate(A, T[s], k);
     const int l = L - m.first(s), nb = T[s].ninj_blk;
     const int t = (l % nb) * 256 + threadIdx.x;
-    if (t < T[s].I.npts * 27) elastic_inject_one<VTI>(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t, l / nb);
+    if (t < T[s].I.ntgt) elastic_inject_one<VTI>(A.next, A, T[s].I, T[s].src, T[s].nt, k, 0, t, l / nb);
 }
 
 // Receivers of local step k in [0, nt-1] read the current field.

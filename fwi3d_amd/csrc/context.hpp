@@ -63,6 +63,7 @@ struct fwi3d_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     fwi3d::DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;
+    std::vector<int> injhost[2];  // host copies of the injection lists (source, receiver tables)
     // shot operator (operator.hip): converted model, reduction scalars, traces at the
     // propagation rate, raw FP64 gradient, synthetic data + misfit of an FWI shot
     fwi3d::DevBuf opmodel, opscal, optrace, opgrad, opsyn;

@@ -477,31 +477,38 @@ __device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, in
 // next += taper(dt^2 inj / rho / scale); mode 1: initial prev = 0.5 dt^2 inj(src[0]) / rho.
 template <bool VTI>
 __device__ __forceinline__ void elastic_inject_one(float* __restrict__ out3, const EArgs& A, const InjTab& I,
-                                                   const float* __restrict__ src, int nt, int it, int mode, int t, int comp) {
+                                                   const float* __restrict__ src, int nt, int it, int mode, int j, int comp) {
     float* __restrict__ out = out3 + comp * A.g.n;
     const Geom& g = A.g;
     const float dt = A.dt;
-    const int r = t / 27, qq = t % 27;
-    const int a = qq / 9, bb = (qq / 3) % 3, c = qq % 3;
-    const int ix = I.xi[3 * r + bb], iy = I.yi[3 * r + a], iz = I.zi[3 * r + c];
+    const int e0 = I.tbeg[j], e1 = I.tbeg[j + 1];
+    int t = I.tlst[e0];
+    const int ix = I.xi[3 * (t / 27) + (t % 27 / 3) % 3], iy = I.yi[3 * (t / 27) + t % 27 / 9],
+              iz = I.zi[3 * (t / 27) + t % 3];
     const int p = g.at(ix, iy, iz);
-    const float v = inj_value(I, src, nt, it, comp, r, a, bb, c);
-    float add;
-    if (mode == 1) {
-        add = v * (0.5f * dt * dt / A.rho[p]);
-    } else {
-        const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
-        add = dt * dt * v / A.rho[p];
-        add = add / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, comp, dt);
-        add = taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+    float val = out[p];
+    for (int e = e0; e < e1; e++) {
+        t = I.tlst[e];
+        const int r = t / 27, q = t % 27;
+        const float v = inj_value(I, src, nt, it, comp, r, q / 9, (q / 3) % 3, q % 3);
+        float add;
+        if (mode == 1) {
+            add = v * (0.5f * dt * dt / A.rho[p]);
+        } else {
+            const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
+            add = dt * dt * v / A.rho[p];
+            add = add / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, comp, dt);
+            add = taper_apply(add, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+        }
+        val += add;
     }
-    atomicAdd(out + p, add);
+    out[p] = val;
 }
 template <bool VTI>
 __global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, const float* __restrict__ src, int nt, int it,
                                int mode) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < I.npts * 27) elastic_inject_one<VTI>(out3, A, I, src, nt, it, mode, t, blockIdx.y);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < I.ntgt) elastic_inject_one<VTI>(out3, A, I, src, nt, it, mode, j, blockIdx.y);
 }
 
 // Imaging condition of nl_we_op_e / nl_we_op_vti ::compute_gradients

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -76,14 +77,55 @@ void to_padded_on(hipStream_t stream, float* dst, const float* src, int k, const
 }
 void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g) { to_padded_on(c->stream, dst, src, k, g); }
 
-// Device bytes of one injector's tables (ints then floats).
+// Device bytes of one injector's tables (ints, floats, then the per-target lists).
 size_t inj_bytes(const fwi3d_injector* h) {
     const size_t nw = h->kind ? 6 : 3;
-    return 3 * (size_t)3 * h->npts * sizeof(int) + 3 * (2 * nw * h->npts) * sizeof(float) + 256;
+    return 3 * (size_t)3 * h->npts * sizeof(int) + 3 * (2 * nw * h->npts) * sizeof(float) +
+           (2 * (size_t)27 * h->npts + 1) * sizeof(int) + 256;
 }
 
-// Upload injector tables to device memory at `base` (inj_bytes(h) bytes).
-InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h) {
+// Per-target contribution lists of the injection (InjTab::tbeg / tlst) into hv:
+// [tbeg (ntgt+1)][tlst (27 npts)]; returns ntgt. Targets in order of first appearance,
+// each target's contributions in increasing t = 27 r + 9 a + 3 b + c.
+int inj_targets(const fwi3d_injector* h, std::vector<int>& hv) {
+    const int nc = 27 * h->npts;
+    std::vector<std::pair<long long, int>> key(nc);
+    for (int t = 0; t < nc; t++) {
+        const int r = t / 27, q = t % 27;
+        const long long iy = h->yind[3 * r + q / 9], ix = h->xind[3 * r + (q / 3) % 3], iz = h->zind[3 * r + q % 3];
+        key[t] = {(iy << 42) | (ix << 21) | iz, t};
+    }
+    std::vector<int> first(nc);
+    {   // order targets by their first contribution
+        std::vector<std::pair<long long, int>> srt(key);
+        std::stable_sort(srt.begin(), srt.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (int k = 0; k < nc;) {
+            int e = k;
+            while (e < nc && srt[e].first == srt[k].first) e++;
+            for (int q = k; q < e; q++) first[srt[q].second] = srt[k].second;
+            k = e;
+        }
+    }
+    std::vector<int> order(nc);
+    for (int t = 0; t < nc; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return first[x] < first[y]; });
+    hv.assign((size_t)nc + 1, 0);
+    std::vector<int> beg;
+    beg.reserve(nc + 1);
+    for (int k = 0; k < nc; k++) {
+        if (k == 0 || first[order[k]] != first[order[k - 1]]) beg.push_back(k);
+        hv[k] = order[k];
+    }
+    beg.push_back(nc);
+    std::vector<int> out(beg);
+    out.insert(out.end(), hv.begin(), hv.begin() + nc);
+    hv.swap(out);
+    return (int)beg.size() - 1;
+}
+
+// Upload injector tables to device memory at `base` (inj_bytes(h) bytes). hv keeps the
+// host copy of the per-target lists alive until the stream has consumed it.
+InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h, std::vector<int>& hv) {
     InjTab t{};
     t.npts = h->npts;
     t.kind = h->kind;
@@ -95,6 +137,8 @@ InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h) {
     float* xw = (float*)(zi + ni);
     float* yw = xw + nw;
     float* zw = yw + nw;
+    int* tl = (int*)(zw + nw);
+    t.ntgt = 0;
     if (h->npts > 0) {
         HIP_OK(hipMemcpyAsync(xi, h->xind, ni * sizeof(int), hipMemcpyHostToDevice, stream));
         HIP_OK(hipMemcpyAsync(yi, h->yind, ni * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -102,15 +146,19 @@ InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h) {
         HIP_OK(hipMemcpyAsync(xw, h->xw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
         HIP_OK(hipMemcpyAsync(yw, h->yw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
         HIP_OK(hipMemcpyAsync(zw, h->zw, nw * sizeof(float), hipMemcpyHostToDevice, stream));
+        t.ntgt = inj_targets(h, hv);
+        HIP_OK(hipMemcpyAsync(tl, hv.data(), hv.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     }
     t.xi = xi; t.yi = yi; t.zi = zi; t.xw = xw; t.yw = yw; t.zw = zw;
+    t.tbeg = tl;
+    t.tlst = tl + t.ntgt + 1;
     return t;
 }
 
 // Upload injector tables into the context buffer `buf`.
-InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h) {
+InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h, std::vector<int>& hv) {
     buf.ensure(inj_bytes(h));
-    return upload_inj_to(c->stream, buf.as<char>(), h);
+    return upload_inj_to(c->stream, buf.as<char>(), h, hv);
 }
 
 void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char* what) {
@@ -220,11 +268,34 @@ void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ych
         }
 }
 
-// ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero
+// ---- snapshot store: slot k holds forward step k*sub; unwritten slots read as zero.
+// A store is `slots` slots of ncomp x n values (FP32, or 16-bit codes with one scale
+// word per slot). The context owns the store of the single-shot path; a batch owns one
+// per gradient shot.
+struct SnapDesc {
+    char* data = nullptr;
+    unsigned* scale = nullptr;
+    int slots = 0, ncomp = 0, fmt = FWI3D_SNAP_F32;
+    long long n = 0;
+    size_t slot_elems() const { return (size_t)ncomp * n; }
+    size_t esz() const { return fmt == FWI3D_SNAP_F32 ? 4 : 2; }
+};
+
+size_t snap_store_bytes(int slots, int ncomp, long long n, int fmt) {
+    return (size_t)slots * ncomp * n * (fmt == FWI3D_SNAP_F32 ? 4 : 2);
+}
+
+SnapDesc ctx_snap(const fwi3d_ctx* c) {
+    SnapDesc s;
+    s.data = c->snaps.as<char>();
+    s.scale = c->snapsc.as<unsigned>();
+    s.slots = c->snap_slots; s.ncomp = c->snap_ncomp; s.fmt = c->snap_fmt; s.n = c->snap_n;
+    return s;
+}
+
 void snap_begin(fwi3d_ctx* c, int physics, int ncomp, const fwi3d_param* par, const Geom& g) {
     const int slots = 1 + par->nt / par->sub;
-    const int esz = par->snap_precision == FWI3D_SNAP_F32 ? 4 : 2;
-    const size_t bytes = (size_t)slots * ncomp * g.n * esz;
+    const size_t bytes = snap_store_bytes(slots, ncomp, g.n, par->snap_precision);
     c->snaps.ensure(bytes);
     c->snapsc.ensure((size_t)slots * sizeof(unsigned));
     HIP_OK(hipMemsetAsync(c->snaps.p, 0, bytes, c->stream));
@@ -242,38 +313,74 @@ void snap_check(const fwi3d_ctx* c, int physics, const fwi3d_param* par, const G
     REQUIRE(c->snap_fmt == par->snap_precision, "adjoint snap_precision differs from the forward propagation's");
 }
 
-size_t snap_slot_elems(const fwi3d_ctx* c) { return (size_t)c->snap_ncomp * c->snap_n; }
-
-void snap_put(fwi3d_ctx* c, int k, const float* u) {
-    const size_t ne = snap_slot_elems(c);
-    unsigned* mb = c->snapsc.as<unsigned>() + k;
-    if (c->snap_fmt == FWI3D_SNAP_F32) {
-        HIP_OK(hipMemcpyAsync(c->snaps.as<float>() + (size_t)k * ne, u, ne * sizeof(float), hipMemcpyDeviceToDevice,
-                              c->stream));
+// Store field u (ncomp x n values) in slot k.
+void snap_put(hipStream_t st, const SnapDesc& S, int k, const float* u) {
+    const size_t ne = S.slot_elems();
+    unsigned* mb = S.scale + k;
+    if (S.fmt == FWI3D_SNAP_F32) {
+        HIP_OK(hipMemcpyAsync((float*)S.data + (size_t)k * ne, u, ne * sizeof(float), hipMemcpyDeviceToDevice, st));
         return;
     }
-    void* dst = c->snaps.as<char>() + (size_t)k * ne * 2;
+    void* dst = S.data + (size_t)k * ne * 2;
     const unsigned grid = (unsigned)std::min<size_t>(2048, (ne / 4 + 255) / 256 + 1);
-    hipLaunchKernelGGL(snap_maxabs, dim3(grid), dim3(256), 0, c->stream, u, mb, (long long)ne);
-    if (c->snap_fmt == FWI3D_SNAP_BF16)
-        hipLaunchKernelGGL(snap_store<SNAP_BF16>, dim3(2048), dim3(256), 0, c->stream, u, dst, (const unsigned*)mb, (long long)ne);
+    hipLaunchKernelGGL(snap_maxabs, dim3(grid), dim3(256), 0, st, u, mb, (long long)ne);
+    if (S.fmt == FWI3D_SNAP_BF16)
+        hipLaunchKernelGGL(snap_store<SNAP_BF16>, dim3(2048), dim3(256), 0, st, u, dst, (const unsigned*)mb, (long long)ne);
     else
-        hipLaunchKernelGGL(snap_store<SNAP_F16>, dim3(2048), dim3(256), 0, c->stream, u, dst, (const unsigned*)mb, (long long)ne);
+        hipLaunchKernelGGL(snap_store<SNAP_F16>, dim3(2048), dim3(256), 0, st, u, dst, (const unsigned*)mb, (long long)ne);
 }
 
 template <int FMT>
-SnapView<FMT> snap_view(const fwi3d_ctx* c, int k) {
-    if (k < 0 || k >= c->snap_slots) return SnapView<FMT>{nullptr, nullptr};
-    const size_t esz = FMT == SNAP_F32 ? 4 : 2;
-    return SnapView<FMT>{c->snaps.as<char>() + (size_t)k * snap_slot_elems(c) * esz, c->snapsc.as<unsigned>() + k};
+SnapView<FMT> snap_view(const SnapDesc& S, int k) {
+    if (k < 0 || k >= S.slots) return SnapView<FMT>{nullptr, nullptr};
+    return SnapView<FMT>{S.data + (size_t)k * S.slot_elems() * S.esz(), S.scale + k};
 }
 
-// Call f(std::integral_constant<int, FMT>) for the context's snapshot format.
+// Call f(std::integral_constant<int, FMT>) for a snapshot format.
 template <class Fn>
-void with_snap_fmt(const fwi3d_ctx* c, Fn&& f) {
-    if (c->snap_fmt == FWI3D_SNAP_BF16) f(std::integral_constant<int, SNAP_BF16>{});
-    else if (c->snap_fmt == FWI3D_SNAP_F16) f(std::integral_constant<int, SNAP_F16>{});
+void with_snap_fmt(int fmt, Fn&& f) {
+    if (fmt == FWI3D_SNAP_BF16) f(std::integral_constant<int, SNAP_BF16>{});
+    else if (fmt == FWI3D_SNAP_F16) f(std::integral_constant<int, SNAP_F16>{});
     else f(std::integral_constant<int, SNAP_F32>{});
+}
+
+// One imaging-condition event of compute_gradients (we_op.cpp:1555-1605) at snapshot
+// index itl: the adjoint field adjf against forward slots itl-1, itl, itl+1.
+void grad_event_acoustic(hipStream_t st, const SnapDesc& S, int itl, int nt, int sub, float dt, const float* adjf,
+                         const float* M, const Geom& g, double* gK, double* gB) {
+    const float sdt = (float)sub * dt;
+    const int last = nt / sub;
+    const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
+    dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+    with_snap_fmt(S.fmt, [&](auto tag) {
+        constexpr int FMT = decltype(tag)::value;
+        // a slot past the store reads as zero (the reference dereferences a null slot there)
+        const SnapView<FMT> fn = snap_view<FMT>(S, itl), fm = itl > 0 ? snap_view<FMT>(S, itl - 1) : fn,
+                            fp = snap_view<FMT>(S, itl + 1);
+        hipLaunchKernelGGL(acoustic_grad<FMT>, grid, dim3(64), 0, st, fm, fn, fp, adjf, M, g, sdt, branch, gK, gB);
+    });
+}
+
+// Elastic / VTI event (we_op.cpp:488-565, 1108-1216); lagf is the previous adjoint field
+// (the derivatives of the lambda/mu terms lag one step, as in the reference).
+void grad_event_elastic(hipStream_t st, bool vti, const SnapDesc& S, int itl, int nt, int sub, float dt,
+                        const float* adjf, const float* lagf, const EArgs& A, double* grad, long long nunp) {
+    const float sdt = (float)sub * dt;
+    const int last = nt / sub;
+    const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
+    const Geom& g = A.g;
+    dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+    with_snap_fmt(S.fmt, [&](auto tag) {
+        constexpr int FMT = decltype(tag)::value;
+        const SnapView<FMT> fn = snap_view<FMT>(S, itl), fm = itl > 0 ? snap_view<FMT>(S, itl - 1) : fn,
+                            fp = snap_view<FMT>(S, itl + 1);
+        if (vti)
+            hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+                               grad, grad + nunp, grad + 2 * nunp);
+        else
+            hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+                               grad, grad + nunp, grad + 2 * nunp);
+    });
 }
 
 void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const float* d_src, float* d_rcv,
@@ -289,9 +396,9 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     to_padded(c, M, d_model, 2, g);
     const Tapers tp = setup_tapers(c, par, g);
 
-    InjTab I = upload_inj(c, c->injtab, inj);
+    InjTab I = upload_inj(c, c->injtab, inj, c->injhost[0]);
     InjTab E{};
-    if (!adj) E = upload_inj(c, c->exttab, ext);
+    if (!adj) E = upload_inj(c, c->exttab, ext, c->injhost[1]);
 
     const int nt = par->nt, sub = par->sub;
     const float dt = par->dt;
@@ -313,35 +420,25 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
     // one-shot table for the time-step kernels (rotation from the step index)
     BShot T1h{};
+    const int nthr = 256;
+    const int nbi = (I.ntgt + nthr - 1) / nthr;
     T1h.phys = FWI3D_ACOUSTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = n; T1h.F = F; T1h.g = g; T1h.a = A;
     T1h.nshell_blk = (nshell + 255) / 256;
+    T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     const BShot* T1 = upload_shot1(c, T1h);
 
-    const int nthr = 256;
-    const int nbi = (I.npts * 27 + nthr - 1) / nthr;
-    if (I.npts > 0)
+    if (I.ntgt > 0)
         hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, prev, A, I, d_src, nt, 0, 1);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     double* gK = d_grad;
     double* gB = d_grad ? d_grad + nunp : nullptr;
     auto grad_event = [&](int itl, const float* adjf) {
-        const float sdt = (float)sub * dt;
-        const int last = nt / sub;
-        const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-        dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
-        with_snap_fmt(c, [&](auto tag) {
-            constexpr int FMT = decltype(tag)::value;
-            // a slot past the store reads as zero (the reference dereferences a null slot there)
-            const SnapView<FMT> fn = snap_view<FMT>(c, itl), fm = itl > 0 ? snap_view<FMT>(c, itl - 1) : fn,
-                                fp = snap_view<FMT>(c, itl + 1);
-            hipLaunchKernelGGL(acoustic_grad<FMT>, grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, (const float*)M, g, sdt,
-                               branch, gK, gB);
-        });
+        grad_event_acoustic(c->stream, ctx_snap(c), itl, nt, sub, dt, adjf, (const float*)M, g, gK, gB);
     };
 
     for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0) snap_put(c, it / sub, curr);
+        if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, curr);
         if (!adj && E.npts > 0)
             hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it);
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
@@ -354,11 +451,11 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         timed(c, 2, [&] {
             hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, it);
         });
-        if (I.npts > 0)
-            hipLaunchKernelGGL(acoustic_inject, dim3(nbi), dim3(nthr), 0, c->stream, next, A, I, d_src, nt, it, 0);
+        if (I.ntgt > 0)
+            hipLaunchKernelGGL(acoustic_inject_batch, dim3(nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
         float* t = prev; prev = curr; curr = next; next = t;
     }
-    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c, (nt - 1) / sub, curr);
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, curr);
     if (!adj && E.npts > 0)
         hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1);
     if (adj && d_grad && sub > 0) grad_event(0, curr);
@@ -416,9 +513,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     A.g = g;
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
     A.aux = c->aux.as<float>();
-    InjTab I = upload_inj(c, c->injtab, inj);
+    InjTab I = upload_inj(c, c->injtab, inj, c->injhost[0]);
     InjTab E{};
-    if (!adj) E = upload_inj(c, c->exttab, ext);
+    if (!adj) E = upload_inj(c, c->exttab, ext, c->injhost[1]);
 
     const int nt = par->nt, sub = par->sub;
     const float dt = par->dt;
@@ -432,9 +529,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     float* next = F + 6 * n;
 
     const int nthr = 256;
-    const int nbi = (I.npts * 27 + nthr - 1) / nthr;
+    const int nbi = (I.ntgt + nthr - 1) / nthr;
     auto inject = [&](float* out, int it, int mode) {
-        if (I.npts == 0) return;
+        if (I.ntgt == 0) return;
         if (vti)
             hipLaunchKernelGGL(elastic_inject<true>, dim3(nbi, 3), dim3(nthr), 0, c->stream, out, A, I, d_src, nt, it, mode);
         else
@@ -453,25 +550,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     T1h.phys = vti ? FWI3D_VTI : FWI3D_ELASTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = 3 * n; T1h.F = F; T1h.g = g;
     T1h.ef = A; T1h.ei = AI;
     T1h.nshell_blk = (nshell + 255) / 256;
+    T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     const BShot* T1 = upload_shot1(c, T1h);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
-        const float sdt = (float)sub * dt;
-        const int last = nt / sub;
-        const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-        dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
-        with_snap_fmt(c, [&](auto tag) {
-            constexpr int FMT = decltype(tag)::value;
-            const SnapView<FMT> fn = snap_view<FMT>(c, itl), fm = itl > 0 ? snap_view<FMT>(c, itl - 1) : fn,
-                                fp = snap_view<FMT>(c, itl + 1);
-            if (vti)
-                hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt,
-                                   branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
-            else
-                hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64), 0, c->stream, fm, fn, fp, adjf, lagf, A, sdt,
-                                   branch, d_grad, d_grad + nunp, d_grad + 2 * nunp);
-        });
+        grad_event_elastic(c->stream, vti, ctx_snap(c), itl, nt, sub, dt, adjf, lagf, A, d_grad, nunp);
     };
     auto extract = [&](const float* u, int it) {
         if (adj || E.npts == 0) return;
@@ -479,7 +563,7 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
 
     for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0) snap_put(c, it / sub, curr);
+        if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, curr);
         extract(curr, it);
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
@@ -491,10 +575,15 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
             if (TX == 8) step_elastic<8, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
             else step_elastic<4, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
         }
-        inject(next, it, 0);
+        // the time-step injection runs the batch kernel over the one-shot table (the same
+        // code as a batched shot)
+        if (I.ntgt > 0) {
+            if (vti) hipLaunchKernelGGL(elastic_inject_batch<true>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
+            else hipLaunchKernelGGL(elastic_inject_batch<false>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
+        }
         float* t = prev; prev = curr; curr = next; next = t;
     }
-    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c, (nt - 1) / sub, curr);
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, curr);
     extract(curr, nt - 1);
     if (adj && d_grad && sub > 0) grad_event(0, curr, prev);
     HIP_OK(hipGetLastError());

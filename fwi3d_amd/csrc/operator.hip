@@ -774,3 +774,5 @@ int fwi3d_analyze_model(fwi3d_op_param* par, float* model, const fwi3d_shot* s) 
 }
 
 }  // extern "C"
+
+#include "serve.hpp"

@@ -234,9 +234,11 @@ int fwi3d_lbfgs_run(fwi3d_ctx *ctx, const fwi3d_lbfgs_param *par, long long n, f
  * grid size join a batch between time steps and advance together, each time step of the
  * whole batch being one launch per kernel class; a shot leaves after its last sample.
  * Per shot the arguments mean what they mean for fwi3d_propagate_device with adj = 0
- * (converted model, source, receiver buffer [ncomp][nrcv][nt]), and par->sub must be 0
- * (no snapshot store). The caller keeps d_model, d_src and d_rcv alive until
- * fwi3d_batch_query reports the shot done. Each shot's traces equal those of
+ * (converted model, source, receiver buffer [ncomp][nrcv][nt]). With par->sub > 0 the
+ * shot also fills a snapshot store (par->snap_precision), which outlives the shot until
+ * an adjoint shot consumes it (fwi3d_batch_add_adjoint) or fwi3d_batch_release drops it.
+ * The caller keeps d_model, d_src and d_rcv alive until fwi3d_batch_query reports the
+ * shot done. Each shot's traces, and each adjoint shot's gradient, equal those of
  * fwi3d_propagate_device for the same inputs. Calls on one batch, and on the context it
  * was created on, must not overlap. */
 typedef struct fwi3d_batch fwi3d_batch;
@@ -250,9 +252,27 @@ long long fwi3d_batch_footprint(int physics, int nx, int ny, int nz, const fwi3d
 int fwi3d_batch_add(fwi3d_batch *b, int physics, const float *d_model, const float *d_src, float *d_rcv,
                     const fwi3d_injector *inj, const fwi3d_injector *ext, const fwi3d_param *par, int nx, int ny,
                     int nz, float dx, float dy, float dz, int *id);
+/* Device bytes of the snapshot store of a forward shot with imaging interval sub > 0
+ * (held from its admission until its adjoint shot leaves); -1 for invalid arguments. */
+long long fwi3d_batch_snap_footprint(int physics, int nx, int ny, int nz, int nt, int sub, int snap_precision);
+/* Admit the adjoint shot of forward shot fwd_id (which must have left the batch):
+ * fwi3d_propagate_device with adj = 1 -- d_src [ncomp][inj->npts][nt] injected at the
+ * receivers inj, the imaging condition (+=) into d_grad64 [ng][ny][nx][nz] from fwd_id's
+ * snapshots, par (nt, sub, snap_precision) and grid equal to the forward shot's. The
+ * snapshot store is freed when the adjoint shot leaves. Replaces the adjoint
+ * propagate(adj=true, grad) call of apply_jacobianT (src/cpp/we_op.cpp:1022-1031). */
+int fwi3d_batch_add_adjoint(fwi3d_batch *b, int fwd_id, int physics, const float *d_model, const float *d_src,
+                            const fwi3d_injector *inj, double *d_grad64, const fwi3d_param *par, int nx, int ny, int nz,
+                            float dx, float dy, float dz, int *id);
+/* Drop the snapshot store of forward shot fwd_id without an adjoint shot. */
+int fwi3d_batch_release(fwi3d_batch *b, int fwd_id);
 /* Enqueue up to nsteps time steps of every shot in the batch (asynchronous on the
  * context's stream); *nactive = shots still in the batch afterwards. */
 int fwi3d_batch_step(fwi3d_batch *b, int nsteps, int *nactive);
+/* Ids of the shots that left the batch during the last fwi3d_batch_step call (their
+ * work is enqueued on the context's stream, so work the caller enqueues there afterwards
+ * sees their results): *n ids, at most cap written to ids. */
+int fwi3d_batch_retired(const fwi3d_batch *b, int *ids, int cap, int *n);
 /* *state = 1 when shot id's traces are complete, 0 while it runs, -1 for an unknown id. */
 int fwi3d_batch_query(fwi3d_batch *b, int id, int *state);
 int fwi3d_batch_wait(fwi3d_batch *b);
@@ -264,6 +284,61 @@ int fwi3d_batch_status(const fwi3d_batch *b, int *nshots, long long *resident_by
  * and, if cand_ms is not NULL, the measured milliseconds per step of the 4 candidates. */
 int fwi3d_batch_autotune(fwi3d_batch *b, int window);
 int fwi3d_batch_tiles(const fwi3d_batch *b, int *ychunk, float *cand_ms);
+
+/* ---------------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, whose drivers run one operator per
+ * shot to completion: we_modeling.cpp:84-112, optimization.cpp:33-190). A server thread
+ * owns a batch on the context and keeps it full: queued requests are admitted in
+ * arrival order as soon as their device footprint fits the HBM budget and one of
+ * max_active slots is free, between two time steps of the shots already running
+ * (continuous batching); model analysis and source resampling of new requests overlap
+ * the running shots on a side stream. Each request computes what the one-shot entry
+ * points compute for it alone: FWI3D_REQ_MODEL = fwi3d_apply_forward_device (add = 0),
+ * FWI3D_REQ_GRADIENT = fwi3d_fwi_shot_device. While a server exists, the context belongs
+ * to it: the caller makes no other calls on that context.
+ * ------------------------------------------------------------------------------ */
+enum fwi3d_request_kind { FWI3D_REQ_MODEL = 0, FWI3D_REQ_GRADIENT = 1 };
+
+typedef struct {
+    int kind;                  /* fwi3d_request_kind */
+    fwi3d_op_param par;        /* operator parameters; physics from par.nmodels */
+    fwi3d_shot shot;           /* geometry (the position arrays are copied at submission) */
+    const float *d_model;      /* raw model [nmodels][ny][nx][nz] */
+    const float *d_src;        /* source wavelets [nscomp][nt_data] */
+    const float *d_obs;        /* GRADIENT: observed data [nrcomp][nrcv][nt_data] */
+    float *d_data;             /* MODEL: data [nrcomp][nrcv][nt_data] (overwritten);
+                                  GRADIENT: residual syn - obs, or NULL */
+    double *d_grad64;          /* GRADIENT: FP64 gradient [min(3,nmodels)][ny][nx][nz] (+=) */
+} fwi3d_request;
+
+typedef struct {
+    long long submitted, completed, failed;
+    int queued, active;        /* requests waiting for admission / admitted and not done */
+    long long hbm_in_use;      /* device bytes reserved by admitted requests */
+    long long hbm_peak;
+    int peak_active;
+    long long steps;           /* batch time steps run */
+} fwi3d_server_counters;
+
+typedef struct fwi3d_server fwi3d_server;
+/* hbm_budget: device bytes the admitted requests may hold together (their shots in the
+ * batch, snapshot stores, converted models, traces and FP64 raw gradients). */
+int fwi3d_server_create(fwi3d_ctx *ctx, long long hbm_budget, int max_active, fwi3d_server **out);
+/* Queue a request (returns at once; device buffers stay owned by the caller and must
+ * stay alive until the request is done). */
+int fwi3d_server_submit(fwi3d_server *s, const fwi3d_request *r, int *id);
+/* *state: 0 queued, 1 running, 2 done, -1 failed (fwi3d_server_error). misfit: the FP64
+ * misfit contribution of a done gradient request; latency_s: submission to completion
+ * (so far, if not done); queue_s: submission to admission. Any output may be NULL. */
+int fwi3d_server_poll(fwi3d_server *s, int id, int *state, double *misfit, double *latency_s, double *queue_s);
+/* Block until request id (id < 0: every submitted request) is done or failed. */
+int fwi3d_server_wait(fwi3d_server *s, int id);
+const char *fwi3d_server_error(fwi3d_server *s, int id);
+/* The operator parameters of a done request after analyzeModel (dt, nt, sub, ...). */
+int fwi3d_server_result(fwi3d_server *s, int id, fwi3d_op_param *par);
+int fwi3d_server_stats(fwi3d_server *s, fwi3d_server_counters *st);
+/* Finish every submitted request, stop the thread, free the server. */
+int fwi3d_server_destroy(fwi3d_server *s);
 
 #ifdef __cplusplus
 }

@@ -132,23 +132,125 @@ def test_batch_mix_equals_alone_and_single_path():
         eng.close()
 
 
-def test_batch_rejects_snapshots_and_reports_footprint():
+GRAD_SHOTS = [  # physics, shape, nt, seed, sub, snapshot precision
+    ("elastic", (30, 34, 38), 100, 12, 4, "f32"),   # (nt-1) % sub != 0: a zero slot, as in the reference
+    ("acoustic", (40, 44, 48), 91, 11, 3, "f32"),
+    ("vti", (28, 26, 40), 81, 13, 4, "bf16"),
+]
+
+
+def test_batch_gradient_shots_equal_single_path():
+    """Forward shots with a snapshot store and their adjoint shots, mixed with a plain
+    forward shot, give the traces and FP64 gradients of fwi3d_propagate_device
+    (forward with sub > 0, then adjoint on the same context) bit for bit."""
+    import torch
+    dev = torch.device("cuda:0")
+    eng = F.Engine(0)
+    try:
+        cases = []
+        for ph, shape, nt, seed, sub, fmt in GRAD_SHOTS:
+            s = make_shot(ph, shape, nt, seed)
+            s["par"] = F.make_param(nt, DT, sub=sub, bc=s["par"].bc, taper=s["par"].taper, snap_precision=fmt)
+            rng = np.random.default_rng(seed + 100)
+            nc = F.n_comp(F.PHYSICS[ph])
+            asrc = (rng.standard_normal((nc, s["ext"].npts, nt)) * np.hanning(nt)).astype(np.float32)
+            m = torch.from_numpy(s["model"]).to(dev)
+            src = torch.from_numpy(s["src"]).to(dev)
+            a = torch.from_numpy(asrc).to(dev)
+            ng = F.n_grad(F.PHYSICS[ph])
+            rcv1 = torch.zeros((nc, s["ext"].npts, nt), dtype=torch.float32, device=dev)
+            g1 = torch.zeros((ng,) + shape, dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            print(f"  single path: {ph} {shape} nt={nt} sub={sub} {fmt}", flush=True)
+            eng.propagate_device(ph, m.data_ptr(), src.data_ptr(), rcv1.data_ptr(), s["inj"], s["ext"], s["par"], shape,
+                                 (H, H, H))
+            eng.propagate_device(ph, m.data_ptr(), a.data_ptr(), 0, s["ext"], s["inj"], s["par"], shape, (H, H, H),
+                                 adj=True, grad64_ptr=g1.data_ptr())
+            torch.cuda.synchronize()
+            cases.append(dict(s=s, m=m, src=src, a=a, rcv1=rcv1, g1=g1, rcv2=torch.zeros_like(rcv1),
+                              g2=torch.zeros_like(g1)))
+        plain = make_shot("acoustic", (36, 30, 70), 75, 4)
+        pm = torch.from_numpy(plain["model"]).to(dev)
+        psrc = torch.from_numpy(plain["src"]).to(dev)
+        prcv = torch.zeros((1, plain["ext"].npts, 75), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        print("  batch: 3 gradient shots + 1 plain shot", flush=True)
+        b = F.Batch(eng)
+        fwd = {}
+        for k, c in enumerate(cases):
+            s = c["s"]
+            sid = b.add(s["physics"], c["m"].data_ptr(), c["src"].data_ptr(), c["rcv2"].data_ptr(), s["inj"], s["ext"],
+                        s["par"], s["shape"], (H, H, H))
+            fwd[sid] = k
+            fp = F.Batch.snap_footprint(s["physics"], s["shape"], s["par"].nt, s["par"].sub, s["par"].snap_precision)
+            assert fp > 0
+        b.add("acoustic", pm.data_ptr(), psrc.data_ptr(), prcv.data_ptr(), plain["inj"], plain["ext"], plain["par"],
+              plain["shape"], (H, H, H))
+        # an adjoint shot may not start before its forward shot has left
+        k0 = next(iter(fwd))
+        c0 = cases[fwd[k0]]
+        with pytest.raises(RuntimeError, match="has not left"):
+            b.add_adjoint(k0, c0["s"]["physics"], c0["m"].data_ptr(), c0["a"].data_ptr(), c0["s"]["ext"],
+                          c0["g2"].data_ptr(), c0["s"]["par"], c0["s"]["shape"], (H, H, H))
+        adj_started, guard = 0, 0
+        while b.step(1) > 0 or adj_started < len(cases):
+            guard += 1
+            assert guard < 1000, "batch did not drain"
+            for sid in b.retired():
+                if sid in fwd:
+                    c = cases[fwd[sid]]
+                    s = c["s"]
+                    b.add_adjoint(sid, s["physics"], c["m"].data_ptr(), c["a"].data_ptr(), s["ext"], c["g2"].data_ptr(),
+                                  s["par"], s["shape"], (H, H, H))
+                    adj_started += 1
+        b.wait()
+        assert b.status()[0] == 0 and b.status()[1] == 0  # shots and snapshot stores all freed
+        b.close()
+        for c in cases:
+            s = c["s"]
+            r1, r2 = c["rcv1"].cpu().numpy(), c["rcv2"].cpu().numpy()
+            g1, g2 = c["g1"].cpu().numpy(), c["g2"].cpu().numpy()
+            print(f"  {s['physics']} {s['shape']}: traces max|d| {np.abs(r1 - r2).max():.1e}, gradient max|d| "
+                  f"{np.abs(g1 - g2).max():.1e} (|g| max {np.abs(g1).max():.2e})", flush=True)
+            assert np.abs(g1).max() > 0
+            assert np.array_equal(r1, r2)
+            assert np.array_equal(g1, g2)
+    finally:
+        eng.close()
+
+
+def test_batch_snapshot_store_errors_and_footprint():
     import torch
     dev = torch.device("cuda:0")
     s = make_shot("elastic", (24, 24, 24), 40, 7)
     fp = F.Batch.footprint("elastic", s["shape"], s["inj"], s["ext"])
     ny, nx, nz = s["shape"]
     assert fp >= 3 * 3 * ny * nx * 64 * 4  # three 3-component wavefields at a 64-float z pitch
+    sfp = F.Batch.snap_footprint("elastic", s["shape"], 40, 4, "f32")
+    assert sfp >= 11 * 3 * ny * nx * 64 * 4  # 1 + nt/sub slots of 3 components
+    assert F.Batch.snap_footprint("elastic", s["shape"], 40, 4, "bf16") < sfp
     eng = F.Engine(0)
     try:
         b = F.Batch(eng)
         m = torch.from_numpy(s["model"]).to(dev)
         src = torch.from_numpy(s["src"]).to(dev)
         rcv = torch.zeros((3, s["ext"].npts, 40), dtype=torch.float32, device=dev)
+        g = torch.zeros((3, ny, nx, nz), dtype=torch.float64, device=dev)
         par = F.make_param(40, DT, sub=4)
-        with pytest.raises(RuntimeError, match="sub must be 0"):
-            b.add("elastic", m.data_ptr(), src.data_ptr(), rcv.data_ptr(), s["inj"], s["ext"], par, s["shape"], (H, H, H))
-        assert b.status()[0] == 0 and b.query(12345) == -1
+        sid = b.add("elastic", m.data_ptr(), src.data_ptr(), rcv.data_ptr(), s["inj"], s["ext"], par, s["shape"],
+                    (H, H, H))
+        assert b.status()[1] >= fp + sfp
+        with pytest.raises(RuntimeError, match="no snapshot store"):
+            b.add_adjoint(sid + 7, "elastic", m.data_ptr(), src.data_ptr(), s["ext"], g.data_ptr(), par, s["shape"],
+                          (H, H, H))
+        b.run()
+        assert b.query(sid) == 1 and b.query(12345) == -1
+        assert b.status()[0] == 0 and b.status()[1] >= sfp  # the store outlives its forward shot
+        with pytest.raises(RuntimeError, match="does not match"):
+            b.add_adjoint(sid, "elastic", m.data_ptr(), src.data_ptr(), s["ext"], g.data_ptr(),
+                          F.make_param(40, DT, sub=2), s["shape"], (H, H, H))
+        b.release(sid)
+        assert b.status()[1] == 0
         b.close()
     finally:
         eng.close()
