@@ -253,6 +253,209 @@ def run_batch(args, torch, F, W):
     eng.close()
 
 
+def _smooth(torch, x, sigma):
+    """Gaussian filter (sigma in cells, truncated at 4 sigma, edge values repeated) along
+    the three axes of x, on the device."""
+    r = int(4 * sigma + 0.5)
+    k = torch.exp(-0.5 * (torch.arange(-r, r + 1, device=x.device, dtype=torch.float32) / sigma) ** 2)
+    k = (k / k.sum()).view(1, 1, -1)
+    for axis in range(3):
+        y = x.movedim(axis, -1)
+        shp = y.shape
+        y = torch.nn.functional.pad(y.reshape(-1, 1, shp[-1]), (r, r), mode="replicate")
+        x = torch.nn.functional.conv1d(y, k).reshape(shp).movedim(-1, axis).contiguous()
+    return x
+
+
+def serve_inputs(torch, F, W, req, dev):
+    """Device inputs of one serving request, generated from its seed: the acoustic
+    configuration's linear-gradient model, the elastic one's five layers with a +5%
+    Gaussian anomaly (seeded centre within +-100 m), the VTI one's linear vp with smooth
+    random epsilon/delta; Gardner density; sources and receivers as in those configs."""
+    n, nt, dt, h = req.n, req.nt, req.dt, 0.01
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(req.seed)
+    z = torch.arange(n, device=dev, dtype=torch.float64)
+    c = (n - 1) * h / 2
+    gardner = lambda vp: (0.31 * (1000.0 * vp.double()) ** 0.25).float()
+    if req.physics == "acoustic":
+        vp = (1.5 + 1.0 * z / (n - 1)).float().expand(n, n, n).contiguous()
+        raw = torch.stack([vp, gardner(vp)])
+        w = W.ricker(15.0, nt, dt)[None, :]
+        sxyz, rxyz, nm, mt, ns = (c, c, 0.05), W._surface_grid(c, c, 0.02, 10, 0.05), 2, 0, 1
+    elif req.physics == "elastic":
+        vals = torch.linspace(2.0, 3.5, 5, device=dev, dtype=torch.float64)
+        bounds = [round(k * n / 5) for k in range(6)]
+        vz = torch.empty(n, device=dev, dtype=torch.float64)
+        for k in range(5):
+            vz[bounds[k]:bounds[k + 1]] = vals[k]
+        off = (torch.rand(3, generator=gen, device=dev, dtype=torch.float64) - 0.5) * 0.2
+        x = torch.arange(n, device=dev, dtype=torch.float64) * h
+        r2 = ((x[:, None, None] - c - off[0]) ** 2 + (x[None, :, None] - c - off[1]) ** 2 +
+              (x[None, None, :] - c - off[2]) ** 2)
+        vp = (vz[None, None, :] * (1 + 0.05 * torch.exp(-r2 / 0.3 ** 2))).float()
+        raw = torch.stack([vp, (vp / 1.8).float(), gardner(vp)])
+        w = W.ricker(12.0, nt, dt)
+        nrec = min(40, int(((n - 1) * h - 0.04) / 0.05) + 1)
+        sxyz, rxyz, nm, mt, ns = (c, c, 0.03), W._surface_grid(c, c, 0.02, nrec, 0.05), 3, 1, 6
+        w = np.stack([w, w, w] + [0 * w] * 3)
+    else:
+        vp = (1.8 + 2.2 * z / (n - 1)).float().expand(n, n, n).contiguous()
+
+        def field(scale):
+            f = _smooth(torch, torch.randn((n, n, n), generator=gen, device=dev, dtype=torch.float32), 5.0)
+            return (scale * (f - f.min()) / (f.max() - f.min()).clamp_min(1e-30)).float()
+
+        eps, dlt = field(0.2), field(0.1)
+        raw = torch.stack([vp, (vp / 2).float(), gardner(vp), dlt, eps, torch.zeros_like(vp)])
+        w = W.ricker(15.0, nt, dt)
+        sxyz, rxyz, nm, mt, ns = (c, c, 0.02), W._surface_grid(c, c, 0.02, 16, (n - 40) * h / 16), 6, 1, 6
+        w = np.stack([w, w, w] + [0 * w] * 3)
+    geo = F.Shot((n, n, n), (h, h, h), (0.0, 0.0, 0.0), nt, dt, sxyz, rxyz)
+    grad = req.kind == "gradient"
+    par = dict(nmodels=nm, dt=dt, sub=4 if grad else 0, mt=mt, bc=(1, 2, 2, 2, 2, 2), taper=(0, 20, 20, 20, 20, 20),
+               snap_precision="bf16")
+    nrc = 1 if nm == 2 else 3
+    out = dict(req=req, geo=geo, par=par, model=raw.contiguous(),
+               src=torch.from_numpy(np.ascontiguousarray(np.asarray(w, np.float32).reshape(ns, nt))).to(dev),
+               data=torch.zeros((nrc, geo.nrcv, nt), dtype=torch.float32, device=dev))
+    if grad:
+        out["obs"] = torch.zeros_like(out["data"])  # residual = synthetic data
+        out["grad"] = torch.zeros((min(3, nm), n, n, n), dtype=torch.float64, device=dev)
+    return out
+
+
+def run_serve(args, torch, F, W):
+    """Serving stream: --requests seeded requests (60% acoustic modelling n^3, n in
+    100..200; 30% elastic shot gradients, n in 128..256, BF16 snapshots every 4 steps;
+    10% VTI modelling 160^3; nt in 1000..3000). (1) plain: one request after another
+    through the one-shot entry points; (2) served at saturation: every request queued at
+    once into the serving loop (capacity); (3) served under a Poisson stream at 0.7 x that
+    capacity: p50/p99 latency. Results of (2) are compared with (1) bit for bit."""
+    dev = torch.device("cuda:0")
+    reqs = W.serve_mix(args.requests)
+    t0 = time.perf_counter()
+    inputs = [serve_inputs(torch, F, W, r, dev) for r in reqs]
+    torch.cuda.synchronize()
+    print(f"# {len(reqs)} requests generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cells = sum(W.cell_updates(r) for r in reqs)
+    ngrad = sum(r.kind == "gradient" for r in reqs)
+
+    def plain_one(eng, q, data, grad):
+        par = F.op_param(**q["par"])
+        if q["req"].kind == "model":
+            F.apply_forward_device(eng, par, q["geo"], False, q["model"].data_ptr(), q["src"].data_ptr(), data.data_ptr())
+        else:
+            F.fwi_shot_device(eng, par, q["geo"], q["model"].data_ptr(), q["src"].data_ptr(), q["obs"].data_ptr(),
+                              grad.data_ptr(), data.data_ptr())
+
+    # (1) plain, one request at a time (a small warm-up request first, untimed)
+    eng = F.Engine(0)
+    warm = serve_inputs(torch, F, W, W.Request(-1, "gradient", "elastic", 40, 50, 0.0008, 1), dev)
+    plain_one(eng, warm, warm["data"], warm["grad"])
+    plain_out, lat_plain = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in inputs:
+        ts = time.perf_counter()
+        d = torch.zeros_like(q["data"])
+        g = torch.zeros_like(q["grad"]) if "grad" in q else None
+        plain_one(eng, q, d, g)
+        plain_out.append((d, g))
+        lat_plain.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    eng.close()
+    torch.cuda.empty_cache()
+    print(f"# plain: {t_plain:.2f} s", file=sys.stderr, flush=True)
+
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(min(args.hbm_budget_gib * 2**30, free - 16 * 2**30))
+
+    def serve(arrivals):
+        eng = F.Engine(0)
+        srv = F.Server(eng, budget, args.max_active)
+        for q in inputs:
+            if "grad" in q:
+                q["grad"].zero_()
+        torch.cuda.synchronize()
+        ids = []
+        t0 = time.perf_counter()
+        for k, q in enumerate(inputs):
+            if arrivals is not None:
+                wait = t0 + arrivals[k] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            par = F.op_param(**q["par"])
+            if q["req"].kind == "model":
+                ids.append(srv.submit(F.REQ_MODEL, par, q["geo"], q["model"].data_ptr(), q["src"].data_ptr(),
+                                      data_ptr=q["data"].data_ptr()))
+            else:
+                ids.append(srv.submit(F.REQ_GRADIENT, par, q["geo"], q["model"].data_ptr(), q["src"].data_ptr(),
+                                      data_ptr=q["data"].data_ptr(), obs_ptr=q["obs"].data_ptr(),
+                                      grad64_ptr=q["grad"].data_ptr()))
+        srv.wait()
+        elapsed = time.perf_counter() - t0
+        polls = [srv.poll(i) for i in ids]
+        stats = srv.stats()
+        errors = [srv.error(i) for i, p in zip(ids, polls) if p[0] != 2]
+        srv.close()
+        eng.close()
+        if errors:
+            raise RuntimeError(f"serving requests failed: {errors[:3]}")
+        return elapsed, [p[2] for p in polls], [p[3] for p in polls], stats
+
+    # (2) saturation: every request queued at t = 0
+    t_srv, lat_sat, _, st_sat = serve(None)
+    print(f"# served (saturation): {t_srv:.2f} s, counters {st_sat}", file=sys.stderr, flush=True)
+    same = all(torch.equal(q["data"], d) and (g is None or torch.equal(q["grad"], g))
+               for q, (d, g) in zip(inputs, plain_out))
+    del plain_out
+    # (3) Poisson stream at 0.7 x the measured capacity
+    rate = 0.7 * len(reqs) / t_srv
+    arr = W.poisson_arrivals(len(reqs), rate)
+    t_poi, lat, qwait, st_poi = serve(arr)
+    print(f"# served (Poisson, {rate:.3f} req/s): {t_poi:.2f} s", file=sys.stderr, flush=True)
+    pct = lambda v, q: round(float(np.percentile(v, q)), 4)
+    kinds = {}
+    for r, l in zip(reqs, lat):
+        kinds.setdefault(f"{r.kind}_{r.physics}", []).append(l)
+    out = {
+        "metric": METRIC,
+        "value": round(cells / t_srv / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(t_srv * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (models generated per request from its seed; zero observed data, so the residual is the "
+                "synthetic data)",
+        "config": {"workload": f"serving stream: {len(reqs)} seeded requests (60% acoustic modelling n^3 n~U{{100..200}}, "
+                               f"30% elastic shot gradients n~U{{128..256}} with BF16 snapshots every 4 steps, 10% VTI "
+                               f"modelling 160^3; nt~U{{1000..3000}}), continuous batching (fwi3d_server)",
+                   "requests": len(reqs), "gradients": ngrad, "hbm_budget_gib": round(budget / 2**30, 1),
+                   "max_active": args.max_active},
+        "shot_gradients_per_s": {"served": round(ngrad / t_srv, 4), "plain": round(ngrad / t_plain, 4)},
+        "requests_per_s": {"served": round(len(reqs) / t_srv, 4), "plain": round(len(reqs) / t_plain, 4)},
+        "speedup_vs_plain": round(t_plain / t_srv, 3),
+        "plain_gcell_s": round(cells / t_plain / 1e9, 3),
+        "poisson": {"rate_req_per_s": round(rate, 4), "makespan_s": round(t_poi, 2),
+                    "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                    "queue_s": {"p50": pct(qwait, 50), "p99": pct(qwait, 99)},
+                    "latency_p50_by_kind_s": {k: pct(v, 50) for k, v in kinds.items()},
+                    "hbm_peak_gib": round(st_poi["hbm_peak"] / 2**30, 2), "peak_active": st_poi["peak_active"]},
+        "saturation": {"hbm_peak_gib": round(st_sat["hbm_peak"] / 2**30, 2), "peak_active": st_sat["peak_active"],
+                       "latency_s": {"p50": pct(lat_sat, 50), "p99": pct(lat_sat, 99)}},
+        "plain_latency_s": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)},
+        "results_identical_to_plain": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,9 +465,13 @@ def main():
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
-    ap.add_argument("--workload", default="gradient", choices=("gradient", "lbfgs", "batch"),
+    ap.add_argument("--workload", default="gradient", choices=("gradient", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); lbfgs: 256^3 mixed-precision l-BFGS FWI; "
-                         "batch: 32 VTI shots on 3 grids, batched vs one at a time")
+                         "batch: 32 VTI shots on 3 grids, batched vs one at a time; serve: a seeded stream of "
+                         "mixed requests through the serving loop")
+    ap.add_argument("--requests", type=int, default=48, help="requests of the serving stream (--workload serve)")
+    ap.add_argument("--max-active", type=int, default=12, help="serving slots (--workload serve)")
+    ap.add_argument("--hbm-budget-gib", type=float, default=200.0, help="serving HBM budget (--workload serve)")
     ap.add_argument("--iters", type=int, default=10, help="l-BFGS iterations (--workload lbfgs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates the HBM traffic counters)")
@@ -286,6 +493,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args, torch, F, W)
+        return
+    if args.workload == "serve":
+        run_serve(args, torch, F, W)
         return
 
     shot = W.config_b(n=args.n, nt=args.nt)

@@ -186,3 +186,48 @@ def config_c_shot(grid_index, shot_index, nt=1500, dt=0.0008, h=0.01):
     rcv = _surface_grid(cx, cy, 0.02, 16, sp)
     return Shot("vti", vti_convert(*raw), np.stack(raw), src, np.array([[pos[0], pos[1], 0.02]], F32), 1, rcv,
                 (h, h, h), nt, dt, meta={"name": f"vti_{nz}x{nx}x{ny}", "grid": grid_index, "freq": 15.0})
+
+
+# ---------------------------------------------------------------------------------
+# Serving stream: a seeded mix of modelling and gradient requests.
+
+@dataclass
+class Request:
+    """One request of the serving stream; its model is generated from `seed`."""
+    index: int
+    kind: str        # "model" or "gradient"
+    physics: str     # "acoustic", "elastic", "vti"
+    n: int           # cubic grid n^3
+    nt: int
+    dt: float
+    seed: int
+
+
+def serve_mix(count, seed=42):
+    """60% acoustic modelling on n^3 grids, n ~ U{100..200}; 30% elastic shot gradients,
+    n ~ U{128..256}; 10% VTI modelling on 160^3; nt ~ U{1000..3000}. dt 1 ms (acoustic)
+    or 0.8 ms (elastic, VTI), as in the acoustic / elastic / VTI configurations."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        u = rng.random()
+        nt = int(rng.integers(1000, 3001))
+        s = int(rng.integers(0, 2**31 - 1))
+        if u < 0.6:
+            out.append(Request(i, "model", "acoustic", int(rng.integers(100, 201)), nt, 0.001, s))
+        elif u < 0.9:
+            out.append(Request(i, "gradient", "elastic", int(rng.integers(128, 257)), nt, 0.0008, s))
+        else:
+            out.append(Request(i, "model", "vti", 160, nt, 0.0008, s))
+    return out
+
+
+def poisson_arrivals(count, rate, seed=42):
+    """Arrival times (s) of a Poisson stream of `rate` requests per second."""
+    rng = np.random.default_rng(seed + 1)
+    return np.cumsum(rng.exponential(1.0 / rate, count))
+
+
+def cell_updates(req):
+    """Cell-updates of a request: n^3 (nt-1), twice for a gradient (forward + adjoint)."""
+    return float(req.n) ** 3 * (req.nt - 1) * (2 if req.kind == "gradient" else 1)
