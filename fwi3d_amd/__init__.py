@@ -288,7 +288,8 @@ class Engine:
 
     def profile_read(self, kernel=0):
         """(milliseconds, launches) of kernel class 0 (interior time step), 1 (flux
-        pre-pass, elastic/VTI) or 2 (boundary-shell time step)."""
+        pre-pass, elastic/VTI), 2 (boundary-shell time step) or 3 (z-slab time step,
+        elastic/VTI)."""
         ms = C.c_double()
         n = C.c_longlong()
         self._check(lib().fwi3d_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)), "fwi3d_profile_read")

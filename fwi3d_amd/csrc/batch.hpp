@@ -25,7 +25,8 @@
 // Host side of the batch.
 
 struct fwi3d_batch {
-    enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VINJ, C_EXT, NCLS };
+    enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EZS, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VZS, C_VINJ,
+               C_EXT, NCLS };
     struct Shot {
         int id = -1;
         fwi3d::BShot d{};
@@ -190,27 +191,37 @@ void batch_one_step(fwi3d_batch* b) {
     hipStream_t st = b->c->stream;
     const BShot* T = b->tab.as<BShot>();
     const int it = b->it;
+    auto go_on = [&](hipStream_t sq, int k, auto kern, dim3 blk) {
+        if (b->cls_blocks[k] > 0) hipLaunchKernelGGL(kern, dim3(b->cls_blocks[k]), blk, 0, sq, T, batch_map(b, k), it);
+    };
     auto go = [&](int k, auto kern, int threads, dim3 blk) {
-        if (b->cls_blocks[k] > 0) hipLaunchKernelGGL(kern, dim3(b->cls_blocks[k]), blk, 0, st, T, batch_map(b, k), it);
+        go_on(st, k, kern, blk);
         (void)threads;
     };
     go(B::C_EXT, extract_batch, 128, dim3(128));
+    // flux passes, then the interiors, shells and z slabs (disjoint points of each shot's
+    // next field), then the injections
+    if (b->TX == 8) {
+        go(B::C_EFLUX, elastic_flux_batch<8, false>, 0, dim3(64, 8));
+        go(B::C_VFLUX, elastic_flux_batch<8, true>, 0, dim3(64, 8));
+    } else {
+        go(B::C_EFLUX, elastic_flux_batch<4, false>, 0, dim3(64, 4));
+        go(B::C_VFLUX, elastic_flux_batch<4, true>, 0, dim3(64, 4));
+    }
     if (b->TX == 8) {
         go(B::C_AINT, acoustic_interior_batch<8>, 0, dim3(64, 8));
-        go(B::C_EFLUX, elastic_flux_batch<8, false>, 0, dim3(64, 8));
         go(B::C_EINT, elastic_interior_batch<8, false>, 0, dim3(64, 8));
-        go(B::C_VFLUX, elastic_flux_batch<8, true>, 0, dim3(64, 8));
         go(B::C_VINT, elastic_interior_batch<8, true>, 0, dim3(64, 8));
     } else {
         go(B::C_AINT, acoustic_interior_batch<4>, 0, dim3(64, 4));
-        go(B::C_EFLUX, elastic_flux_batch<4, false>, 0, dim3(64, 4));
         go(B::C_EINT, elastic_interior_batch<4, false>, 0, dim3(64, 4));
-        go(B::C_VFLUX, elastic_flux_batch<4, true>, 0, dim3(64, 4));
         go(B::C_VINT, elastic_interior_batch<4, true>, 0, dim3(64, 4));
     }
     go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
     go(B::C_ESHELL, elastic_boundary_batch<false>, 0, dim3(256));
     go(B::C_VSHELL, elastic_boundary_batch<true>, 0, dim3(256));
+    go(B::C_EZS, elastic_zslab_batch<kZSlabTX, false>, 0, dim3(8 * kZSlabTX));
+    go(B::C_VZS, elastic_zslab_batch<kZSlabTX, true>, 0, dim3(8 * kZSlabTX));
     go(B::C_AINJ, acoustic_inject_batch, 0, dim3(256));
     go(B::C_EINJ, elastic_inject_batch<false>, 0, dim3(256));
     go(B::C_VINJ, elastic_inject_batch<true>, 0, dim3(256));
@@ -414,7 +425,10 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             const int ni = tile_grid_yc(d.ei, g, TX, true, yc);
             s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = nf;
             s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = ni;
+            d.nshell_pts = shell_xy_count(g);
+            d.nshell_blk = (d.nshell_pts + 255) / 256;
             s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 3 * d.nshell_blk;
+            s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g);
             s.nblk[grp == 1 ? B::C_EINJ : B::C_VINJ] = 3 * d.ninj_blk;
             A.prev = F; A.curr = F + 3 * n; A.next = F + 6 * n;
             if (d.I.ntgt > 0) {

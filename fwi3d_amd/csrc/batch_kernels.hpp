@@ -23,6 +23,8 @@ struct BShot {
     const float* src;
     float* rcv;
     int nshell_blk;  // shell workgroups of one component (256 points each)
+    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the x and y slabs)
+    ZSlabTiles zs;   // elastic / VTI: tiling of the z-slab kernel
     int ninj_blk;    // injection workgroups of one component (256 targets each)
     int next_blk;    // extraction workgroups of one component (128 receivers)
 };
@@ -113,7 +115,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void e
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
     const int l = L - m.first(s), nb = T[s].nshell_blk;
-    elastic_boundary_block<VTI>(A, l % nb, l / nb);
+    elastic_boundary_block<VTI>(A, l % nb, l / nb, T[s].nshell_pts);
+}
+
+// z slabs of the shell (elastic_zslab_tile), 8 x TXZ threads per workgroup.
+template <int TXZ, bool VTI>
+__global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __restrict__ T, BMap m, int it) {
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s);
+    elastic_zslab_tile<TXZ, VTI>(A, T[s].zs, [&] { return l; });
 }
 
 // ---- source injection into `next` after the step (mode 0), receivers before it.

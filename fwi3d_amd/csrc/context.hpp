@@ -76,7 +76,8 @@ struct fwi3d_ctx {
     bool snap_valid = false;
     int snap_dims[3] = {0, 0, 0};
     // profiling: [0] interior time-step kernel, [1] flux pre-pass (elastic/VTI),
-    // [2] boundary-shell time-step kernel, [3] reserved
+    // [2] boundary-shell time-step kernel, [3] z-slab kernel (elastic/VTI), [4] the whole
+    // time step (its kernels, launched one after another)
     // tiling of the y-marching kernels (fwi3d_tune)
     int tune_ychunk = 0;  // y planes per workgroup (0: default 8)
     int tune_tx = 4;      // x rows per workgroup (4 or 8)
@@ -86,8 +87,9 @@ struct fwi3d_ctx {
     };
     std::vector<Tiles> tuned;
     bool prof = false;
-    double prof_ms[4] = {0, 0, 0, 0};
-    long long prof_n[4] = {0, 0, 0, 0};
+    static constexpr int kProfClasses = 5;  // interior, flux, shell, z slabs, whole time step
+    double prof_ms[kProfClasses] = {};
+    long long prof_n[kProfClasses] = {};
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
     std::vector<int> ev_kind;

@@ -105,9 +105,21 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
         }
     };
     prefetch(y0);
+    // model values at this column of the current plane, loaded one iteration ahead:
+    // lambda, mu [, eps, c13, c66]
+    constexpr int NMQ = VTI ? 5 : 2;
+    float mq[NMQ], mn[NMQ];
+    auto mload = [&](int y, float* m) {
+        const int pp = min(y, ny - 1) * sy + col;
+        m[0] = lamf[pp];
+        m[1] = muf[pp];
+        if (VTI) { m[2] = epsf[pp]; m[3] = c13f[pp]; m[4] = c66f[pp]; }
+    };
+    mload(y0, mq);
     for (int iy = y0; iy < y1; iy++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) q[c][5] = ldy(curr + c * n3, iy + 3);
+        mload(iy + 1, mn);
         const int pl = iy * sy;
         __syncthreads();
 #pragma unroll
@@ -136,20 +148,20 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
                 dyu[c] = dyu[c] / dy;
                 dzu[c] = dzu[c] / dz;
             }
-            const float mu = muf[p];
+            const float mu = mq[1];
             int nT;
             if (!VTI) {
-                aux[p] = lamf[p] * (dxu[0] + dyu[1] + dzu[2]);
+                aux[p] = mq[0] * (dxu[0] + dyu[1] + dzu[2]);
                 nT = 1;
             } else {
-                const float lam = lamf[p], e2 = 1.f + 2.f * epsf[p], c13 = c13f[p], c66 = c66f[p];
+                const float lam = mq[0], e2 = 1.f + 2.f * mq[2], c13 = mq[3], c66 = mq[4];
                 const float c11c = e2 * lam, c12 = e2 * (lam + 2.f * mu) - 2.f * c66;
                 aux[p] = c11c * dxu[0] + c12 * dyu[1] + c13 * dzu[2];
                 aux[n3 + p] = c11c * dyu[1] + c12 * dxu[0] + c13 * dzu[2];
                 aux[2 * n3 + p] = c13 * (dxu[0] + dyu[1]) + lam * dzu[2];
                 nT = 3;
             }
-            const float gxy = VTI ? c66f[p] : mu;
+            const float gxy = VTI ? mq[NMQ - 1] : mu;
             float* __restrict__ F = aux + nT * n3;
             F[FX_YX * n3 + p] = dxu[1] * gxy;  // G * D_x u_y  (outer D_y, component x)
             F[FX_ZX * n3 + p] = dxu[2] * mu;   // mu * D_x u_z (outer D_z, component x)
@@ -162,6 +174,8 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
         for (int c = 0; c < 3; c++) {
             q[c][0] = q[c][1]; q[c][1] = q[c][2]; q[c][2] = q[c][3]; q[c][3] = q[c][4]; q[c][4] = q[c][5];
         }
+#pragma unroll
+        for (int k = 0; k < NMQ; k++) mq[k] = mn[k];
     }
 }
 
@@ -263,10 +277,21 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
         }
     };
     prefetch(y0);
+    // rho and the three prev values at this column of the current plane, loaded one
+    // iteration ahead
+    float rq[4], rn[4];
+    auto rload = [&](int y, float* m) {
+        const int pp = min(y, ny - 1) * sy + col;
+        m[0] = rhof[pp];
+#pragma unroll
+        for (int c = 0; c < 3; c++) m[1 + c] = prev[c * n3 + pp];
+    };
+    rload(y0, rq);
 
     for (int iy = y0; iy < y1; iy++) {
 #pragma unroll
         for (int k = 0; k < NQ; k++) q[k][5] = ldy(qsrc[k], iy + 3);
+        rload(iy + 1, rn);
         const int pl = iy * sy;
         __syncthreads();
 #pragma unroll
@@ -294,7 +319,7 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
                 return d2_interior(SZ(PU + c, -2), SZ(PU + c, -1), SZ(PU + c, 0), SZ(PU + c, 1), SZ(PU + c, 2),
                                    SZ(PB + kb, -2), SZ(PB + kb, -1), SZ(PB + kb, 0), SZ(PB + kb, 1), SZ(PB + kb, 2));
             };
-            const float rho = rhof[p];
+            const float rho = rq[0];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float acc = (d2_factor<VTI>(c, 0) * ih2[0]) * d2(c, d2_bfield<VTI>(c, 0), 0);
@@ -314,7 +339,7 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
                     acc = acc + d1x(PF + 3) / dx;     // D_x F_xz
                     acc = acc + d1y(5 + NB) / dy;     // D_y F_yz
                 }
-                float pv = prev[c * n3 + p];
+                float pv = rq[1 + c];
                 if (ptap) pv = taper_apply(pv, tz2, tx2, ty2, ix, iy, iz);
                 const float nv = dt2 * acc / rho + 2.f * q[c][2] - pv;
                 next[c * n3 + p] = taper_apply(nv, tz2, tx2, ty2, ix, iy, iz);
@@ -324,6 +349,8 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
         for (int k = 0; k < NQ; k++) {
             q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) rq[k] = rn[k];
     }
 }
 
@@ -463,14 +490,198 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
     }
 }
 
-// Shell workgroup b (256 points of the shell enumeration), displacement component c.
+// Shell workgroup b (256 points of the shell enumeration), displacement component c;
+// points from npts on (the z slabs, which elastic_zslab_tile computes) are skipped.
 template <bool VTI>
-__device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, int c) {
+__device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, int c, int npts) {
     int ix, iy, iz;
-    if (!shell_point(b * 256 + threadIdx.x, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
+    const int t = b * 256 + threadIdx.x;
+    if (t >= npts || !shell_point(t, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
     if (c == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);
     else if (c == 1) elastic_boundary_point<VTI, 1>(A, ix, iy, iz);
     else elastic_boundary_point<VTI, 2>(A, ix, iy, iz);
+}
+
+// ------------------------------------------------------------------------------------
+// Pass 2, z slabs: the points within 6 cells of the top or bottom face and at least 6
+// cells inside the x and y faces (a third of the shell at 200^3, and its costliest part
+// as point-wise work: a wave there covers only a few z columns). A y-marching tile of
+// 8 z points (the slab's 6 plus the 2 further points its closure rows read) x TXZ x
+// rows: the x-z plane of every operand is staged in LDS with 2-cell x halos, the y
+// neighbours live in register queues (as in the interior kernel), so the z closure
+// rows, the z-face traction SATs and the interior x / y stencils all read on-chip
+// values. Arithmetic and order of operations are those of elastic_boundary_point.
+// Tiling: ZSlabTiles (the x tiles start at 6, the y chunks at 6 and stop at ny-6).
+struct ZSlabTiles {
+    int ntx, nyc, ychunk;
+};
+
+template <int TXZ, bool VTI, class LFn>
+__device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, const ZSlabTiles Z, const LFn& tile_id) {
+    constexpr int NB = VTI ? 3 : 1;
+    constexpr int NT = VTI ? 3 : 1;
+    constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
+    constexpr int NPL = PF + 4;
+    constexpr int NQ = 3 + NB + 3;
+    __shared__ float s[NPL][TXZ + 4][8];
+    const Geom& g = A.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, sx = g.sx, sy = g.sy, n3 = g.n;
+    const float* __restrict__ curr = A.curr;
+    const float* __restrict__ prev = A.prev;
+    const float* bf[3] = {A.mu, VTI ? A.c66 : A.mu, VTI ? A.c11b : A.mu};
+    const float* Ff = A.aux + NT * n3;
+    const int L = tile_id();
+    const int slab = L & 1, tx = (L >> 1) % Z.ntx, cy = (L >> 1) / Z.ntx;
+    const int lz = threadIdx.x & 7, lx = threadIdx.x >> 3;
+    const int zb = slab ? nz - 8 : 0;  // z of LDS column 0
+    const int iz = zb + lz;
+    const int x0 = 6 + tx * TXZ, ix = x0 + lx;
+    const int y0 = 6 + cy * Z.ychunk, y1 = min(ny - 6, y0 + Z.ychunk);
+    const int col = min(ix, nx - 1) * sx + iz;
+    const bool act = ix < nx - 6 && (slab ? lz >= 2 : lz < 6);
+    const bool hi = slab != 0;
+    const int r = hi ? 7 - lz : lz;  // row counted from the face
+    const bool face = r == 0;
+    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    // x-halo value of this thread: threads 0..31 load rows x0-2, x0-1, x0+TXZ, x0+TXZ+1
+    const int h = threadIdx.x;
+    const bool has_h = h < 32;
+    const int hr = (h >> 3) & 3;
+    const int hrow = hr < 2 ? hr : TXZ + hr;
+    const int hsrc = max(0, min(nx - 1, hr < 2 ? x0 - 2 + hr : x0 + TXZ + hr - 2)) * sx + zb + (h & 7);
+
+    const float* qsrc[NQ];
+#pragma unroll
+    for (int c = 0; c < 3; c++) qsrc[c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) qsrc[3 + k] = bf[k];
+    qsrc[3 + NB] = A.aux + (VTI ? n3 : 0);
+    qsrc[4 + NB] = Ff + FX_YX * n3;
+    qsrc[5 + NB] = Ff + FX_YZ * n3;
+    const float* psrc[NPL];
+#pragma unroll
+    for (int c = 0; c < 3; c++) psrc[PU + c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) psrc[PB + k] = bf[k];
+    psrc[PT] = A.aux;
+    if (VTI) psrc[PT + 1] = A.aux + 2 * n3;
+    psrc[PF + 0] = Ff + FX_ZX * n3;
+    psrc[PF + 1] = Ff + FX_XY * n3;
+    psrc[PF + 2] = Ff + FX_ZY * n3;
+    psrc[PF + 3] = Ff + FX_XZ * n3;
+
+    float q[NQ][6];
+#pragma unroll
+    for (int k = 0; k < NQ; k++)
+#pragma unroll
+        for (int o = 0; o < 5; o++) q[k][o] = ldy(qsrc[k], y0 - 2 + o);
+    float pc[NPL], hv[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; k++) pc[k] = hv[k] = 0.f;
+    // threads without a halo value reload their own point (a cache hit), which keeps the
+    // loads unconditional and pc / hv in registers
+    const int hoff = has_h ? hsrc : col;
+    auto prefetch = [&](int y) {
+        const int pl = min(y, ny - 1) * sy;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            if (k >= 3 + NB) pc[k] = psrc[k][pl + col];
+            hv[k] = psrc[k][pl + hoff];
+        }
+    };
+    prefetch(y0);
+    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
+
+    for (int iy = y0; iy < y1; iy++) {
+#pragma unroll
+        for (int k = 0; k < NQ; k++) q[k][5] = ldy(qsrc[k], iy + 3);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            s[k][lx + 2][lz] = (k < 3 + NB) ? q[k][2] : pc[k];
+            if (has_h) s[k][hrow][h & 7] = hv[k];
+        }
+        prefetch(iy + 1);
+        __syncthreads();
+        if (act) {
+            const int p = iy * sy + ix * sx + iz;
+            auto SX = [&](int k, int o) { return s[k][lx + 2 + o][lz]; };
+            auto FZ = [&](int k, int j) { return s[k][lx + 2][hi ? 7 - j : j]; };  // j-th point from the face
+            auto d1x = [&](int k) { return D1C1 * (SX(k, 1) - SX(k, -1)) + D1C2 * (SX(k, 2) - SX(k, -2)); };
+            auto d1y = [&](int k) { return D1C1 * (q[k][3] - q[k][1]) + D1C2 * (q[k][4] - q[k][0]); };
+            auto d1z = [&](int k) {
+                if (r >= 4)
+                    return D1C1 * (s[k][lx + 2][lz + 1] - s[k][lx + 2][lz - 1]) +
+                           D1C2 * (s[k][lx + 2][lz + 2] - s[k][lx + 2][lz - 2]);
+                return d1_closure_at(!hi, r, [&](int j) { return FZ(k, j); });
+            };
+            auto d2 = [&](int c, int kb, int a) -> float {
+                if (a == 0)
+                    return d2_interior(SX(PU + c, -2), SX(PU + c, -1), SX(PU + c, 0), SX(PU + c, 1), SX(PU + c, 2),
+                                       SX(PB + kb, -2), SX(PB + kb, -1), SX(PB + kb, 0), SX(PB + kb, 1), SX(PB + kb, 2));
+                if (a == 1)
+                    return d2_interior(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4], q[3 + kb][0], q[3 + kb][1],
+                                       q[3 + kb][2], q[3 + kb][3], q[3 + kb][4]);
+                return d2_rows(r, [&](int j) { return FZ(PU + c, j); }, [&](int k) { return FZ(PB + kb, k); });
+            };
+            auto sats = [&](int k) {  // outward S (times h) at the face point
+                return c_sat_s[0] * FZ(k, 0) + c_sat_s[1] * FZ(k, 1) + c_sat_s[2] * FZ(k, 2) + c_sat_s[3] * FZ(k, 3);
+            };
+            const float rho = A.rho[p];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2(c, d2_bfield<VTI>(c, 0), 0);
+                acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2(c, d2_bfield<VTI>(c, 1), 1);
+                acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2(c, d2_bfield<VTI>(c, 2), 2);
+                if (c == 0) {
+                    acc = acc + d1x(PT) / g.dx;
+                    acc = acc + d1y(4 + NB) / g.dy;     // D_y F_yx
+                    acc = acc + d1z(PF + 0) / g.dy;     // D_z F_zx (spacing dy as in the reference)
+                } else if (c == 1) {
+                    acc = acc + d1y(3 + NB) / g.dy;     // D_y T_y
+                    acc = acc + d1x(PF + 1) / g.dx;     // D_x F_xy
+                    acc = acc + d1z(PF + 2) / g.dz;     // D_z F_zy
+                } else {
+                    acc = acc + d1z(VTI ? PT + 1 : PT) / g.dz;
+                    acc = acc + d1x(PF + 3) / g.dx;     // D_x F_xz
+                    acc = acc + d1y(5 + NB) / g.dy;     // D_y F_yz
+                }
+                float pv = prev[c * n3 + p];
+                if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+                const int bc = hi ? A.bc[1] : A.bc[0];  // constant indices (no stack copy of A)
+                if (face && (bc == 1 || bc == 2)) {
+                    // traction SAT of the top / bottom face (elastic_boundary_point, normal axis z)
+                    const float sg = hi ? 1.f : -1.f;
+                    const float adh = 1.0f / (g.dz * SAT_H0);
+                    float term;
+                    if (c == 2) {
+                        const float tang = M.cnorm(2, 0, p) * (d1x(PU + 0) / g.dx) + M.cnorm(2, 1, p) * (d1y(1) / g.dy) +
+                                           M.cnorm(2, 2, p) * (d1z(PU + 2) / g.dz);
+                        term = sg * tang + M.knn(2, p) * sats(PU + 2) / g.dz;
+                        if (bc == 2) term -= M.pimp(2, p) * pv / A.dt;
+                    } else {
+                        const float G = M.gshear(2, c, p);
+                        const float duz = c == 0 ? d1x(PU + 2) / g.dx : d1y(2) / g.dy;
+                        term = sg * G * duz + G * sats(PU + c) / g.dz;
+                        if (bc == 2) term -= 0.5f * sqrtf(M.mu[p] * M.rho[p]) * pv / A.dt;
+                    }
+                    acc -= adh * term;
+                }
+                float nv = A.dt2 * acc / rho + 2.f * q[c][2] - pv;
+                if (face && bc == 2) {
+                    // e_scale of a z-face point away from the x and y faces: only this face
+                    // absorbs (normal component: the P speed, tangential: the S speed)
+                    const float v = c == 2 ? sqrtf((M.lam[p] + 2.f * M.mu[p]) / M.rho[p]) : sqrtf(M.mu[p] / M.rho[p]);
+                    nv = nv / (1.f + (0.f + v * A.dt / (2.f * g.dz * SAT_H0)));
+                }
+                A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
+        }
+    }
 }
 
 // Source injection, all three displacement components (blockIdx.y):

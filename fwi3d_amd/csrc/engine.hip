@@ -174,7 +174,7 @@ void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char*
 // Event-bracketed launch of a step kernel when profiling. Events come from a pool
 // that is created once and reused, so the timed region records but never creates.
 template <class F>
-void timed(fwi3d_ctx* c, int kind, F&& launch) {
+void timed_on(fwi3d_ctx* c, int kind, hipStream_t st, F&& launch) {
     if (!c->prof) {
         launch();
         return;
@@ -186,11 +186,18 @@ void timed(fwi3d_ctx* c, int kind, F&& launch) {
             c->ev.push_back(e);
         }
     }
-    HIP_OK(hipEventRecord(c->ev[c->ev_used], c->stream));
-    launch();
-    HIP_OK(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
-    c->ev_kind.push_back(kind);
+    // reserve the pair first: the launch may time nested work (a whole step around its
+    // kernels), which takes the pairs after this one
+    const size_t i0 = c->ev_used;
     c->ev_used += 2;
+    c->ev_kind.push_back(kind);
+    HIP_OK(hipEventRecord(c->ev[i0], st));
+    launch();
+    HIP_OK(hipEventRecord(c->ev[i0 + 1], st));
+}
+template <class F>
+void timed(fwi3d_ctx* c, int kind, F&& launch) {
+    timed_on(c, kind, c->stream, launch);
 }
 
 void drain_events(fwi3d_ctx* c) {
@@ -256,6 +263,20 @@ int tile_grid_yc(Args& A, const Geom& g, int TX, bool interior, int ychunk) {
     A.nyc = (ey + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;  // 0 when the interior is empty (an axis of 12 points)
 }
+// z-slab kernel of the elastic / VTI shell: x rows per workgroup and y planes per chunk.
+constexpr int kZSlabTX = 32;
+constexpr int kZSlabYChunk = 3;
+// Its tiling (0 workgroups when the slabs have no points inside the x / y slabs).
+int zslab_tiles(ZSlabTiles& Z, const Geom& g) {
+    Z.ychunk = kZSlabYChunk;
+    Z.ntx = (g.nx - 12 + kZSlabTX - 1) / kZSlabTX;
+    Z.nyc = (g.ny - 12 + Z.ychunk - 1) / Z.ychunk;
+    return 2 * Z.ntx * Z.nyc;
+}
+// Shell points left to the point-wise shell kernel once the z slabs are tiled: the
+// y slabs, then the x slabs (the first part of shell_point's enumeration).
+int shell_xy_count(const Geom& g) { return shell_count(6, g.nx, g.ny, g.nz) - (g.ny - 12) * (g.nx - 12) * 12; }
+
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
 // grid on the context, else the context's settings (fwi3d_tune).
 void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk) {
@@ -444,12 +465,14 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        if (nblk > 0) timed(c, 0, [&] {
-            if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, it);
-            else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, it);
-        });
-        timed(c, 2, [&] {
-            hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, it);
+        timed(c, 4, [&] {
+            if (nblk > 0) timed(c, 0, [&] {
+                if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, it);
+                else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, it);
+            });
+            timed(c, 2, [&] {
+                hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, it);
+            });
         });
         if (I.ntgt > 0)
             hipLaunchKernelGGL(acoustic_inject_batch, dim3(nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
@@ -474,7 +497,7 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 // The one-shot table T carries the flux pass's tiling (whole grid, nflux workgroups)
 // and the interior's (nint workgroups, 0 when the interior is empty).
 template <int TX, bool VTI>
-void step_elastic(fwi3d_ctx* c, const BShot* T, int it, int nflux, int nint, int nshell_blk) {
+void step_elastic(fwi3d_ctx* c, const BShot* T, int it, int nflux, int nint, int nshell_blk, int nzslab) {
     timed(c, 1, [&] {
         hipLaunchKernelGGL((elastic_flux_batch<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, T, BMap{}, it);
     });
@@ -485,6 +508,11 @@ void step_elastic(fwi3d_ctx* c, const BShot* T, int it, int nflux, int nint, int
     timed(c, 2, [&] {
         hipLaunchKernelGGL(elastic_boundary_batch<VTI>, dim3(3 * nshell_blk), dim3(256), 0, c->stream, T, BMap{}, it);
     });
+    if (nzslab > 0)
+        timed(c, 3, [&] {
+            hipLaunchKernelGGL((elastic_zslab_batch<kZSlabTX, VTI>), dim3(nzslab), dim3(8 * kZSlabTX), 0, c->stream, T,
+                               BMap{}, it);
+        });
 }
 
 void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, const float* d_src, float* d_rcv,
@@ -549,7 +577,10 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     BShot T1h{};
     T1h.phys = vti ? FWI3D_VTI : FWI3D_ELASTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = 3 * n; T1h.F = F; T1h.g = g;
     T1h.ef = A; T1h.ei = AI;
-    T1h.nshell_blk = (nshell + 255) / 256;
+    (void)nshell;
+    T1h.nshell_pts = shell_xy_count(g);
+    T1h.nshell_blk = (T1h.nshell_pts + 255) / 256;
+    const int nzslab = zslab_tiles(T1h.zs, g);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     const BShot* T1 = upload_shot1(c, T1h);
 
@@ -568,13 +599,15 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
         if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
         A.curr = curr; A.prev = prev; A.next = next;
         A.prev_tapered = it > 0 ? 1 : 0;
-        if (vti) {
-            if (TX == 8) step_elastic<8, true>(c, T1, it, nblk, nint, T1h.nshell_blk);
-            else step_elastic<4, true>(c, T1, it, nblk, nint, T1h.nshell_blk);
-        } else {
-            if (TX == 8) step_elastic<8, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
-            else step_elastic<4, false>(c, T1, it, nblk, nint, T1h.nshell_blk);
-        }
+        timed(c, 4, [&] {
+            if (vti) {
+                if (TX == 8) step_elastic<8, true>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+                else step_elastic<4, true>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+            } else {
+                if (TX == 8) step_elastic<8, false>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+                else step_elastic<4, false>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+            }
+        });
         // the time-step injection runs the batch kernel over the one-shot table (the same
         // code as a batched shot)
         if (I.ntgt > 0) {
@@ -798,15 +831,15 @@ int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
 int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps, int* tx_out, int* ychunk_out) {
     if (!c) return FWI3D_EINVAL;
     const bool prof0 = c->prof;
-    double ms0[4];
-    long long n0[4];
-    std::copy(c->prof_ms, c->prof_ms + 4, ms0);
-    std::copy(c->prof_n, c->prof_n + 4, n0);
+    double ms0[fwi3d_ctx::kProfClasses];
+    long long n0[fwi3d_ctx::kProfClasses];
+    std::copy(c->prof_ms, c->prof_ms + fwi3d_ctx::kProfClasses, ms0);
+    std::copy(c->prof_n, c->prof_n + fwi3d_ctx::kProfClasses, n0);
     const int tx0 = c->tune_tx, yc0 = c->tune_ychunk;
     auto restore = [&] {
         c->tune_tx = tx0; c->tune_ychunk = yc0; c->prof = prof0;
-        std::copy(ms0, ms0 + 4, c->prof_ms);
-        std::copy(n0, n0 + 4, c->prof_n);
+        std::copy(ms0, ms0 + fwi3d_ctx::kProfClasses, c->prof_ms);
+        std::copy(n0, n0 + fwi3d_ctx::kProfClasses, c->prof_n);
     };
     try {
         REQUIRE(physics == FWI3D_ACOUSTIC || physics == FWI3D_ELASTIC || physics == FWI3D_VTI, "unknown physics");
@@ -849,11 +882,11 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             c->tune_tx = tx;
             c->tune_ychunk = yc;
             c->prof = true;
-            for (int k = 0; k < 4; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+            for (int k = 0; k < fwi3d_ctx::kProfClasses; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
             const int rc = fwi3d_propagate_device(c, physics, 0, m.as<float>(), m.as<float>(), nullptr, &none, &none,
                                                   nullptr, &par, nx, ny, nz, 0.01f, 0.01f, 0.01f);
             if (rc != FWI3D_OK) throw Err{rc, c->err};
-            return (c->prof_ms[0] + c->prof_ms[1] + c->prof_ms[2]) / steps;
+            return c->prof_ms[4] / steps;  // whole time steps (the kernels overlap)
         };
         run(4, 8);  // warm-up
         int best_tx = 4, best_yc = 8;
@@ -884,7 +917,7 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
     }
     c->prof = enable != 0;
     if (enable) {
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < fwi3d_ctx::kProfClasses; k++) {
             c->prof_ms[k] = 0;
             c->prof_n[k] = 0;
         }
@@ -893,7 +926,7 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
 }
 
 int fwi3d_profile_read(const fwi3d_ctx* c, int kernel, double* ms, long long* n) {
-    if (!c || kernel < 0 || kernel > 3) return FWI3D_EINVAL;
+    if (!c || kernel < 0 || kernel >= fwi3d_ctx::kProfClasses) return FWI3D_EINVAL;
     if (ms) *ms = c->prof_ms[kernel];
     if (n) *n = c->prof_n[kernel];
     return FWI3D_OK;
