@@ -135,11 +135,13 @@ __device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float K
 }
 
 // One boundary-shell point: closures, SATs (faces in the reference's order top,
-// bottom, left, right, front, back), update, scaling, taper.
-__device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b) {
+// bottom, left, right, front, back), update, scaling, taper. Points of the shell
+// enumeration from npts on are skipped (the acoustic step passes the whole shell).
+__device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b, int npts) {
     const Geom& g = A.g;
     int ix, iy, iz;
-    if (!shell_point(b * 256 + threadIdx.x, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
+    const int t = b * 256 + threadIdx.x;
+    if (t >= npts || !shell_point(t, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
     const float* __restrict__ curr = A.curr;
     const float* __restrict__ bb = A.b;
     const float* __restrict__ K = A.K;
@@ -181,6 +183,12 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b)
     nv = nv / acu_scale(g, A.bc, K[p], bb[p], ix, iy, iz, A.dt);
     A.next[p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
 }
+
+// Tiling of the y-marching z-slab kernel of the elastic / VTI shell (elastic.hpp): x
+// tiles from 6, y chunks from 6 to ny-6.
+struct ZSlabTiles {
+    int ntx, nyc, ychunk;
+};
 
 // ------------------------------------------------------------------------------------
 // Point sources / receivers. Tables: ind [npts][3], w [npts][2][nw] (row 0 inject,

@@ -25,8 +25,8 @@
 // Host side of the batch.
 
 struct fwi3d_batch {
-    enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EZS, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VZS, C_VINJ,
-               C_EXT, NCLS };
+    enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EZS, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VZS,
+               C_VINJ, C_EXT, NCLS };
     struct Shot {
         int id = -1;
         fwi3d::BShot d{};
@@ -402,6 +402,7 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             A.dt = dt; A.dt2 = dt * dt;
             A.alpha = 0.2508560249f / par->free_surface_stiffness;
             s.nblk[B::C_AINT] = tile_grid_yc(A, g, TX, true, yc);
+            d.nshell_pts = nshell;
             s.nblk[B::C_ASHELL] = d.nshell_blk;
             s.nblk[B::C_AINJ] = d.ninj_blk;
             A.prev = F; A.curr = F + n; A.next = F + 2 * n;

@@ -23,7 +23,7 @@ struct BShot {
     const float* src;
     float* rcv;
     int nshell_blk;  // shell workgroups of one component (256 points each)
-    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the x and y slabs)
+    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the y and x slabs)
     ZSlabTiles zs;   // elastic / VTI: tiling of the z-slab kernel
     int ninj_blk;    // injection workgroups of one component (256 targets each)
     int next_blk;    // extraction workgroups of one component (128 receivers)
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __re
     if (k < 0 || k > T[s].nt - 2) return;
     AcuArgs A = T[s].a;
     bshot_rotate(A, T[s], k);
-    acoustic_boundary_block(A, L - m.first(s));
+    acoustic_boundary_block(A, L - m.first(s), T[s].nshell_pts);
 }
 
 // Occupancy: 5 waves per SIMD (96 VGPRs, 8 B/lane spilled) measured faster than the

@@ -511,10 +511,8 @@ __device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, in
 // neighbours live in register queues (as in the interior kernel), so the z closure
 // rows, the z-face traction SATs and the interior x / y stencils all read on-chip
 // values. Arithmetic and order of operations are those of elastic_boundary_point.
-// Tiling: ZSlabTiles (the x tiles start at 6, the y chunks at 6 and stop at ny-6).
-struct ZSlabTiles {
-    int ntx, nyc, ychunk;
-};
+// Tiling: ZSlabTiles (acoustic.hpp; the x tiles start at 6, the y chunks at 6 and stop
+// at ny-6).
 
 template <int TXZ, bool VTI, class LFn>
 __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, const ZSlabTiles Z, const LFn& tile_id) {

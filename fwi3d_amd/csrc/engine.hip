@@ -444,6 +444,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     const int nthr = 256;
     const int nbi = (I.ntgt + nthr - 1) / nthr;
     T1h.phys = FWI3D_ACOUSTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = n; T1h.F = F; T1h.g = g; T1h.a = A;
+    T1h.nshell_pts = nshell;
     T1h.nshell_blk = (nshell + 255) / 256;
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     const BShot* T1 = upload_shot1(c, T1h);
