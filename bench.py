@@ -274,10 +274,20 @@ def run_acoustic(args, torch, F, W):
     src = torch.from_numpy(np.ascontiguousarray(shot.src[:, 0, :])).to(dev)
     data = torch.zeros((1, geo.nrcv, nt), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
+    # per-kernel launch times: one untimed shot launched kernel by kernel with HIP events
+    eng.tune(graphs=False)
+    eng.profile(True)
+    F.apply_forward_device(eng, par(), geo, False, model.data_ptr(), src.data_ptr(), data.data_ptr())
+    kern = {name: eng.profile_read(k) for k, name in ((0, "acoustic_interior"), (2, "acoustic_boundary"),
+                                                      (4, "acoustic_time_step_eager"))}
+    eng.profile(False)
+    eng.tune(graphs=True)
     for _ in range(args.warmup):
         F.apply_forward_device(eng, par(), geo, False, model.data_ptr(), src.data_ptr(), data.data_ptr())
     torch.cuda.synchronize()
-    eng.profile(True)
+    # timed region: the time loop replays captured graphs of 16 steps; HIP events around
+    # each replay
+    eng.profile(True, steps_only=True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -286,8 +296,7 @@ def run_acoustic(args, torch, F, W):
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - ts)
     elapsed = time.perf_counter() - t0
-    prof = {name: eng.profile_read(k) for k, name in ((0, "acoustic_interior"), (2, "acoustic_boundary"),
-                                                      (4, "acoustic_time_step"))}
+    prof = dict(kern, acoustic_time_step=eng.profile_read(4))
     eng.profile(False)
     avg_us = {k: ms / max(c, 1) * 1e3 for k, (ms, c) in prof.items()}
     t_step = avg_us.pop("acoustic_time_step") / 1e6
@@ -312,8 +321,9 @@ def run_acoustic(args, torch, F, W):
         "trace_max_abs": float(data.abs().max()),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "acoustic time step = acoustic_interior + acoustic_boundary launches (HIP events around "
-                               "each step)", "step_us": round(t_step * 1e6, 2), "bytes_per_cell": ACOUSTIC_BYTES_PER_CELL,
+                     "kernel": "acoustic time step = acoustic_interior + acoustic_boundary (+ extraction, injection) "
+                               "replayed from a captured graph (HIP events around each 16-step replay)",
+                     "step_us": round(t_step * 1e6, 2), "bytes_per_cell": ACOUSTIC_BYTES_PER_CELL,
                      "cells_per_step": N, "avg_us": {k: round(v, 2) for k, v in avg_us.items()},
                      "note": "a 100^3 step's 20 MB working set is served from the 256 MB Infinity Cache"},
     }
@@ -426,13 +436,17 @@ def run_serve(args, torch, F, W):
     plain_out, lat_plain = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for q in inputs:
+    t_note = t0
+    for k, q in enumerate(inputs):
         ts = time.perf_counter()
         d = torch.zeros_like(q["data"])
         g = torch.zeros_like(q["grad"]) if "grad" in q else None
         plain_one(eng, q, d, g)
         plain_out.append((d, g))
         lat_plain.append(time.perf_counter() - ts)
+        if time.perf_counter() - t_note > 20:
+            t_note = time.perf_counter()
+            print(f"# plain: {k + 1}/{len(inputs)} requests, {t_note - t0:.0f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     eng.close()
@@ -451,11 +465,23 @@ def run_serve(args, torch, F, W):
         torch.cuda.synchronize()
         ids = []
         t0 = time.perf_counter()
+        t_note = [t0]
+
+        def note(tag):
+            if time.perf_counter() - t_note[0] > 20:
+                t_note[0] = time.perf_counter()
+                st = srv.stats()
+                print(f"# {tag}: {st['completed']}/{len(inputs)} done, {st['active']} running, {st['queued']} queued, "
+                      f"{t_note[0] - t0:.0f} s", file=sys.stderr, flush=True)
+
         for k, q in enumerate(inputs):
             if arrivals is not None:
-                wait = t0 + arrivals[k] - time.perf_counter()
-                if wait > 0:
-                    time.sleep(wait)
+                while True:  # sleep until the arrival, reporting progress meanwhile
+                    wait = t0 + arrivals[k] - time.perf_counter()
+                    if wait <= 0:
+                        break
+                    time.sleep(min(wait, 1.0))
+                    note("serving")
             par = F.op_param(**q["par"])
             if q["req"].kind == "model":
                 ids.append(srv.submit(F.REQ_MODEL, par, q["geo"], q["model"].data_ptr(), q["src"].data_ptr(),
@@ -464,6 +490,12 @@ def run_serve(args, torch, F, W):
                 ids.append(srv.submit(F.REQ_GRADIENT, par, q["geo"], q["model"].data_ptr(), q["src"].data_ptr(),
                                       data_ptr=q["data"].data_ptr(), obs_ptr=q["obs"].data_ptr(),
                                       grad64_ptr=q["grad"].data_ptr()))
+        while True:
+            st = srv.stats()
+            if st["completed"] + st["failed"] == st["submitted"]:
+                break
+            time.sleep(0.05)
+            note("serving")
         srv.wait()
         elapsed = time.perf_counter() - t0
         polls = [srv.poll(i) for i in ids]
@@ -608,10 +640,19 @@ def main():
         misfits.append(F.fwi_shot_device(eng, F.op_param(**opkw), geo, model.data_ptr(), src.data_ptr(),
                                          dobs.data_ptr(), grad.data_ptr()))
 
-    for _ in range(args.warmup):
+    # per-kernel launch times: one untimed shot gradient with every kernel bracketed by
+    # HIP events (also the warm-up)
+    eng.profile(True)
+    shot_gradient()
+    torch.cuda.synchronize()
+    kern = {name: eng.profile_read(k)
+            for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"), (3, "elastic_zslab"))}
+    eng.profile(False)
+    for _ in range(max(0, args.warmup - 1)):
         shot_gradient()
     torch.cuda.synchronize()
-    eng.profile(True)
+    # timed region: HIP events around each time step only
+    eng.profile(True, steps_only=True)
     lat = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -622,9 +663,7 @@ def main():
         lat.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = {name: eng.profile_read(k)
-            for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"), (3, "elastic_zslab"),
-                            (4, "elastic_time_step"))}
+    prof = dict(kern, elastic_time_step=eng.profile_read(4))
     eng.profile(False)
 
     g = grad.cpu().numpy()
@@ -676,6 +715,8 @@ def main():
             "cells_per_step": N,
             "avg_us": {k: round(v, 2) for k, v in avg_us.items()},
             "launches": {k: n for k, (ms, n) in prof.items() if k != "elastic_time_step"},
+            "step_timing": "timed region: HIP events around each time step; per-kernel averages from one untimed shot "
+                           "gradient with every kernel bracketed by events",
         },
     }
     if not args.no_cpu_baseline:

@@ -266,12 +266,15 @@ class Engine:
                                           C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device")
 
-    def tune(self, ychunk=None, tx=None):
-        """Tiling of the time-step kernels (fwi3d_tune); results do not depend on it."""
+    def tune(self, ychunk=None, tx=None, graphs=None):
+        """Tiling of the time-step kernels and graph replay of time loops (fwi3d_tune);
+        results do not depend on them."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")
         if tx is not None:
             self._check(lib().fwi3d_tune(self._h, 1, int(tx)), "fwi3d_tune")
+        if graphs is not None:
+            self._check(lib().fwi3d_tune(self._h, 2, int(bool(graphs))), "fwi3d_tune")
 
     def autotune(self, physics, shape, steps=10):
         """Measure every tile shape on this grid (fwi3d_autotune); the fastest is used for
@@ -283,8 +286,10 @@ class Engine:
                     "fwi3d_autotune")
         return tx.value, yc.value
 
-    def profile(self, enable=True):
-        self._check(lib().fwi3d_profile(self._h, int(bool(enable))), "fwi3d_profile")
+    def profile(self, enable=True, steps_only=False):
+        """Start (reset) or stop HIP-event timing; steps_only records whole time steps only."""
+        level = (2 if steps_only else 1) if enable else 0
+        self._check(lib().fwi3d_profile(self._h, level), "fwi3d_profile")
 
     def profile_read(self, kernel=0):
         """(milliseconds, launches) of kernel class 0 (interior time step), 1 (flux

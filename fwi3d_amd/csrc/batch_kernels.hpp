@@ -38,6 +38,26 @@ struct BMap {
     __device__ __forceinline__ int first(int s) const { return start ? start[s] : 0; }
 };
 
+// Time-step index of a launch: `off`, plus the value at `base` when set (a device
+// counter that a captured graph of several steps advances after each replay, so one
+// graph serves every window of the time loop). An int converts to an eager index.
+struct StepIdx {
+    const int* base;
+    int off;
+    __host__ __device__ StepIdx(int v) : base(nullptr), off(v) {}
+    __host__ __device__ StepIdx(const int* b, int o) : base(b), off(o) {}
+    __device__ __forceinline__ int value() const { return base ? *base + off : off; }
+};
+
+// Advance the graph's step counter by k (the last node of a captured window).
+__global__ void advance_step(int* __restrict__ base, int k) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) base[0] += k;
+}
+// Set the step counter (launched eagerly before the first replay).
+__global__ void set_step(int* __restrict__ base, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) base[0] = v;
+}
+
 template <class Args>
 __device__ __forceinline__ void bshot_rotate(Args& A, const BShot& S, int k) {
     A.prev = S.F + (long long)(k % 3) * S.nf;
@@ -51,10 +71,10 @@ __device__ __forceinline__ void bshot_rotate(Args& A, const BShot& S, int k) {
 // Occupancy: 8 waves per SIMD (forced, 58 VGPRs) measured faster than the compiler's
 // 6 waves (75 VGPRs).
 template <int TX>
-__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(8))) void acoustic_interior_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(8))) void acoustic_interior_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     AcuArgs A = T[s].a;
     bshot_rotate(A, T[s], k);
@@ -62,10 +82,10 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(8))) vo
     acoustic_interior_tile<TX>(A, [&] { return l; });
 }
 
-__global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     AcuArgs A = T[s].a;
     bshot_rotate(A, T[s], k);
@@ -76,10 +96,10 @@ __global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __re
 // compiler's 4 waves without spills.
 template <int TX, bool VTI>
 __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) void elastic_flux_batch(
-    const BShot* __restrict__ T, BMap m, int it) {
+    const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
@@ -91,10 +111,10 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) vo
 // (142-157 VGPRs); 5 waves spill heavily and are much slower.
 template <int TX, bool VTI>
 __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) void elastic_interior_batch(
-    const BShot* __restrict__ T, BMap m, int it) {
+    const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     EArgs A = T[s].ei;
     bshot_rotate(A, T[s], k);
@@ -107,10 +127,10 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) vo
 // 20-40 B/lane spilled) measured faster than 5.
 template <bool VTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void elastic_boundary_batch(
-    const BShot* __restrict__ T, BMap m, int it) {
+    const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
@@ -120,10 +140,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void e
 
 // z slabs of the shell (elastic_zslab_tile), 8 x TXZ threads per workgroup.
 template <int TXZ, bool VTI>
-__global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
@@ -132,10 +152,10 @@ __global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __re
 }
 
 // ---- source injection into `next` after the step (mode 0), receivers before it.
-__global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     AcuArgs A = T[s].a;
     bshot_rotate(A, T[s], k);
@@ -144,10 +164,10 @@ __global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, int i
 }
 
 template <bool VTI>
-__global__ void elastic_inject_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ void elastic_inject_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 2) return;
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
@@ -157,10 +177,10 @@ __global__ void elastic_inject_batch(const BShot* __restrict__ T, BMap m, int it
 }
 
 // Receivers of local step k in [0, nt-1] read the current field.
-__global__ void extract_batch(const BShot* __restrict__ T, BMap m, int it) {
+__global__ void extract_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
-    const int k = it - T[s].t0;
+    const int k = it.value() - T[s].t0;
     if (k < 0 || k > T[s].nt - 1) return;
     const int l = L - m.first(s), nb = T[s].next_blk;
     const int r = (l % nb) * 128 + threadIdx.x;

@@ -86,11 +86,20 @@ struct fwi3d_ctx {
         int physics, nx, ny, nz, tx, ychunk;
     };
     std::vector<Tiles> tuned;
+    // time loops without host work between steps replay a captured graph of several steps
+    // (fwi3d_tune FWI3D_TUNE_GRAPHS); the counter the graph's kernels read the step from
+    bool use_graphs = true;
+    fwi3d::DevBuf stepctr;
+    std::vector<hipGraphExec_t> graph_execs;  // instantiated in the current call, freed after it
+    std::vector<hipGraph_t> graphs;
     bool prof = false;
+    bool prof_steps_only = false;  // record only whole-step events (fwi3d_profile level 2)
     static constexpr int kProfClasses = 5;  // interior, flux, shell, z slabs, whole time step
     double prof_ms[kProfClasses] = {};
     long long prof_n[kProfClasses] = {};
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
     std::vector<int> ev_kind;
+    std::vector<int> ev_count;  // time steps covered by each event pair
+    int timed_count = 1;        // time steps the next timed launch covers
 };

@@ -175,7 +175,7 @@ void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char*
 // that is created once and reused, so the timed region records but never creates.
 template <class F>
 void timed_on(fwi3d_ctx* c, int kind, hipStream_t st, F&& launch) {
-    if (!c->prof) {
+    if (!c->prof || (c->prof_steps_only && kind != 4)) {
         launch();
         return;
     }
@@ -191,6 +191,7 @@ void timed_on(fwi3d_ctx* c, int kind, hipStream_t st, F&& launch) {
     const size_t i0 = c->ev_used;
     c->ev_used += 2;
     c->ev_kind.push_back(kind);
+    c->ev_count.push_back(c->timed_count);
     HIP_OK(hipEventRecord(c->ev[i0], st));
     launch();
     HIP_OK(hipEventRecord(c->ev[i0 + 1], st));
@@ -208,10 +209,62 @@ void drain_events(fwi3d_ctx* c) {
         HIP_OK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
         const int k = c->ev_kind[i / 2];
         c->prof_ms[k] += ms;
-        c->prof_n[k] += 1;
+        c->prof_n[k] += c->ev_count[i / 2];
     }
     c->ev_used = 0;
     c->ev_kind.clear();
+    c->ev_count.clear();
+}
+
+// Time steps [it0, it1) of a loop whose step(StepIdx) enqueues kernels only (no host
+// work between steps): a graph of kGraphSteps steps, whose kernels read the step from a
+// device counter that the graph's last node advances, is captured once and replayed;
+// the remainder runs eagerly. The same kernels run either way, so results do not depend
+// on the mode. The graphs live until release_graphs (after the call's last sync).
+constexpr int kGraphSteps = 16;
+template <class StepFn>
+void run_steps(fwi3d_ctx* c, int it0, int it1, StepFn&& step) {
+    const int K = kGraphSteps;
+    const int W = (it1 - it0) / K;
+    if (!c->use_graphs || W < 2) {
+        for (int it = it0; it < it1; it++) step(StepIdx(it));
+        return;
+    }
+    c->stepctr.ensure(256);
+    int* ctr = c->stepctr.as<int>();
+    const bool prof = c->prof;
+    c->prof = false;  // no event records inside the capture
+    hipGraph_t graph = nullptr;
+    try {
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < K; j++) step(StepIdx(ctr, j));
+        hipLaunchKernelGGL(advance_step, dim3(1), dim3(64), 0, c->stream, ctr, K);
+        HIP_OK(hipStreamEndCapture(c->stream, &graph));
+    } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        c->prof = prof;
+        throw;
+    }
+    c->prof = prof;
+    c->graphs.push_back(graph);
+    hipGraphExec_t exec = nullptr;
+    HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    c->graph_execs.push_back(exec);
+    hipLaunchKernelGGL(set_step, dim3(1), dim3(64), 0, c->stream, ctr, it0);
+    c->timed_count = K;
+    for (int w = 0; w < W; w++) timed(c, 4, [&] { HIP_OK(hipGraphLaunch(exec, c->stream)); });
+    c->timed_count = 1;
+    for (int it = it0 + W * K; it < it1; it++) step(StepIdx(it));
+}
+
+// Free the graphs of the current call (its stream work has completed).
+void release_graphs(fwi3d_ctx* c) {
+    for (auto e : c->graph_execs) (void)hipGraphExecDestroy(e);
+    for (auto g : c->graphs) (void)hipGraphDestroy(g);
+    c->graph_execs.clear();
+    c->graphs.clear();
 }
 
 // Upload a one-shot table for the time-step kernels of the single-shot path.
@@ -447,6 +500,7 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     T1h.nshell_pts = nshell;
     T1h.nshell_blk = (nshell + 255) / 256;
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
+    T1h.E = E; T1h.rcv = d_rcv; T1h.next_blk = (E.npts + 127) / 128;
     const BShot* T1 = upload_shot1(c, T1h);
 
     if (I.ntgt > 0)
@@ -458,33 +512,42 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     auto grad_event = [&](int itl, const float* adjf) {
         grad_event_acoustic(c->stream, ctx_snap(c), itl, nt, sub, dt, adjf, (const float*)M, g, gK, gB);
     };
-
-    for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, curr);
+    // fields of local step k: prev = F[k % 3], curr = F[(k + 1) % 3] (the kernels' rotation)
+    auto field = [&](int k) { return F + (long long)(k % 3) * n; };
+    // one time step: receivers read curr, interior and shell write next, sources add to it
+    auto step = [&](StepIdx si) {
         if (!adj && E.npts > 0)
-            hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, it);
-        if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr);
-        A.curr = curr; A.prev = prev; A.next = next;
-        A.prev_tapered = it > 0 ? 1 : 0;
+            hipLaunchKernelGGL(extract_batch, dim3(T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, si);
         timed(c, 4, [&] {
             if (nblk > 0) timed(c, 0, [&] {
-                if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, it);
-                else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, it);
+                if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, si);
+                else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, si);
             });
             timed(c, 2, [&] {
-                hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, it);
+                hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, si);
             });
         });
         if (I.ntgt > 0)
-            hipLaunchKernelGGL(acoustic_inject_batch, dim3(nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
-        float* t = prev; prev = curr; curr = next; next = t;
+            hipLaunchKernelGGL(acoustic_inject_batch, dim3(nbi), dim3(nthr), 0, c->stream, T1, BMap{}, si);
+    };
+
+    if (!fwd_snap && !(adj && d_grad && sub > 0)) {
+        run_steps(c, 0, nt - 1, step);  // no host work between steps
+    } else {
+        for (int it = 0; it < nt - 1; it++) {
+            if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, field(it + 1));
+            if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, field(it + 1));
+            step(StepIdx(it));
+        }
     }
-    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, curr);
+    (void)prev; (void)curr; (void)next;
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, field(nt));
     if (!adj && E.npts > 0)
-        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 1), dim3(128), 0, c->stream, curr, g, E, d_rcv, nt, nt - 1);
-    if (adj && d_grad && sub > 0) grad_event(0, curr);
+        hipLaunchKernelGGL(extract_batch, dim3(T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, StepIdx(nt - 1));
+    if (adj && d_grad && sub > 0) grad_event(0, field(nt));
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
+    release_graphs(c);
     if (fwd_snap) c->snap_valid = true;
 }
 
@@ -498,7 +561,7 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 // The one-shot table T carries the flux pass's tiling (whole grid, nflux workgroups)
 // and the interior's (nint workgroups, 0 when the interior is empty).
 template <int TX, bool VTI>
-void step_elastic(fwi3d_ctx* c, const BShot* T, int it, int nflux, int nint, int nshell_blk, int nzslab) {
+void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint, int nshell_blk, int nzslab) {
     timed(c, 1, [&] {
         hipLaunchKernelGGL((elastic_flux_batch<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, T, BMap{}, it);
     });
@@ -583,45 +646,54 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     T1h.nshell_blk = (T1h.nshell_pts + 255) / 256;
     const int nzslab = zslab_tiles(T1h.zs, g);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
+    T1h.E = E; T1h.rcv = d_rcv; T1h.next_blk = (E.npts + 127) / 128;
     const BShot* T1 = upload_shot1(c, T1h);
 
     const long long nunp = (long long)g.nx * g.ny * g.nz;
     auto grad_event = [&](int itl, const float* adjf, const float* lagf) {
         grad_event_elastic(c->stream, vti, ctx_snap(c), itl, nt, sub, dt, adjf, lagf, A, d_grad, nunp);
     };
-    auto extract = [&](const float* u, int it) {
-        if (adj || E.npts == 0) return;
-        hipLaunchKernelGGL(extract_points, dim3((E.npts + 127) / 128, 3), dim3(128), 0, c->stream, u, g, E, d_rcv, nt, it);
-    };
-
-    for (int it = 0; it < nt - 1; it++) {
-        if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, curr);
-        extract(curr, it);
-        if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0) grad_event((nt - 1 - it) / sub, curr, prev);
-        A.curr = curr; A.prev = prev; A.next = next;
-        A.prev_tapered = it > 0 ? 1 : 0;
+    // fields of local step k: prev = F[k % 3], curr = F[(k + 1) % 3] (the kernels' rotation)
+    auto field = [&](int k) { return F + (long long)(k % 3) * 3 * n; };
+    // one time step: receivers read curr; flux, interior, shell and z slabs write next;
+    // sources add to it (the batch kernels over the one-shot table: a batched shot runs
+    // the same code)
+    auto step = [&](StepIdx si) {
+        if (!adj && E.npts > 0)
+            hipLaunchKernelGGL(extract_batch, dim3(3 * T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, si);
         timed(c, 4, [&] {
             if (vti) {
-                if (TX == 8) step_elastic<8, true>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
-                else step_elastic<4, true>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+                if (TX == 8) step_elastic<8, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
+                else step_elastic<4, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
             } else {
-                if (TX == 8) step_elastic<8, false>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
-                else step_elastic<4, false>(c, T1, it, nblk, nint, T1h.nshell_blk, nzslab);
+                if (TX == 8) step_elastic<8, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
+                else step_elastic<4, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
             }
         });
-        // the time-step injection runs the batch kernel over the one-shot table (the same
-        // code as a batched shot)
         if (I.ntgt > 0) {
-            if (vti) hipLaunchKernelGGL(elastic_inject_batch<true>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
-            else hipLaunchKernelGGL(elastic_inject_batch<false>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, it);
+            if (vti) hipLaunchKernelGGL(elastic_inject_batch<true>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, si);
+            else hipLaunchKernelGGL(elastic_inject_batch<false>, dim3(3 * nbi), dim3(nthr), 0, c->stream, T1, BMap{}, si);
         }
-        float* t = prev; prev = curr; curr = next; next = t;
+    };
+
+    if (!fwd_snap && !(adj && d_grad && sub > 0)) {
+        run_steps(c, 0, nt - 1, step);  // no host work between steps
+    } else {
+        for (int it = 0; it < nt - 1; it++) {
+            if (fwd_snap && it % sub == 0) snap_put(c->stream, ctx_snap(c), it / sub, field(it + 1));
+            if (adj && d_grad && sub > 0 && (nt - 1 - it) % sub == 0 && it != 0)
+                grad_event((nt - 1 - it) / sub, field(it + 1), field(it));
+            step(StepIdx(it));
+        }
     }
-    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, curr);
-    extract(curr, nt - 1);
-    if (adj && d_grad && sub > 0) grad_event(0, curr, prev);
+    (void)prev; (void)curr; (void)next;
+    if (fwd_snap && (nt - 1) % sub == 0) snap_put(c->stream, ctx_snap(c), (nt - 1) / sub, field(nt));
+    if (!adj && E.npts > 0)
+        hipLaunchKernelGGL(extract_batch, dim3(3 * T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, StepIdx(nt - 1));
+    if (adj && d_grad && sub > 0) grad_event(0, field(nt), field(nt - 1));
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
+    release_graphs(c);
     if (fwd_snap) c->snap_valid = true;
 }
 
@@ -826,19 +898,24 @@ int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
         c->tune_tx = value;
         return FWI3D_OK;
     }
+    if (param == FWI3D_TUNE_GRAPHS) {
+        if (value != 0 && value != 1) return fail(c, Err{FWI3D_EINVAL, "graphs must be 0 or 1"});
+        c->use_graphs = value != 0;
+        return FWI3D_OK;
+    }
     return fail(c, Err{FWI3D_EINVAL, "unknown tuning parameter"});
 }
 
 int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps, int* tx_out, int* ychunk_out) {
     if (!c) return FWI3D_EINVAL;
-    const bool prof0 = c->prof;
+    const bool prof0 = c->prof, steps_only0 = c->prof_steps_only;
     double ms0[fwi3d_ctx::kProfClasses];
     long long n0[fwi3d_ctx::kProfClasses];
     std::copy(c->prof_ms, c->prof_ms + fwi3d_ctx::kProfClasses, ms0);
     std::copy(c->prof_n, c->prof_n + fwi3d_ctx::kProfClasses, n0);
     const int tx0 = c->tune_tx, yc0 = c->tune_ychunk;
     auto restore = [&] {
-        c->tune_tx = tx0; c->tune_ychunk = yc0; c->prof = prof0;
+        c->tune_tx = tx0; c->tune_ychunk = yc0; c->prof = prof0; c->prof_steps_only = steps_only0;
         std::copy(ms0, ms0 + fwi3d_ctx::kProfClasses, c->prof_ms);
         std::copy(n0, n0 + fwi3d_ctx::kProfClasses, c->prof_n);
     };
@@ -883,6 +960,7 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             c->tune_tx = tx;
             c->tune_ychunk = yc;
             c->prof = true;
+            c->prof_steps_only = true;
             for (int k = 0; k < fwi3d_ctx::kProfClasses; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
             const int rc = fwi3d_propagate_device(c, physics, 0, m.as<float>(), m.as<float>(), nullptr, &none, &none,
                                                   nullptr, &par, nx, ny, nz, 0.01f, 0.01f, 0.01f);
@@ -917,6 +995,7 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
         return fail(c, e);
     }
     c->prof = enable != 0;
+    c->prof_steps_only = enable == 2;
     if (enable) {
         for (int k = 0; k < fwi3d_ctx::kProfClasses; k++) {
             c->prof_ms[k] = 0;

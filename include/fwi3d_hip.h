@@ -96,16 +96,21 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
                            const fwi3d_param *par, int nx, int ny, int nz, float dx, float dy, float dz);
 
 /* Kernel timing with HIP events on the context's stream, for roofline accounting
- * (no counterpart in the reference). enable=1 starts collecting (and resets), 0 stops.
+ * (no counterpart in the reference). enable=1 starts collecting every class below (and
+ * resets), enable=2 only whole time steps (class 4: fewer event records in the timed
+ * work), 0 stops.
  * read() returns the summed milliseconds and launch count since the last enable of
  * kernel class `kernel`: 0 = the interior time-step kernel, 1 = the elastic/VTI flux
  * pre-pass, 2 = the boundary-shell time-step kernel, 3 = the elastic/VTI z-slab kernel
  * (the top and bottom 6 planes away from the x and y faces), 4 = the whole time step. */
 /* Tiling of the y-marching time-step kernels, per context (no counterpart in the
  * reference; the tile shapes an autotuner picks): FWI3D_TUNE_YCHUNK = y planes per
- * workgroup (0: default 8), FWI3D_TUNE_TX = x rows per workgroup (4 or 8). Results do
- * not depend on the tiling. */
-enum fwi3d_tune_param { FWI3D_TUNE_YCHUNK = 0, FWI3D_TUNE_TX = 1 };
+ * workgroup (0: default 8), FWI3D_TUNE_TX = x rows per workgroup (4 or 8).
+ * FWI3D_TUNE_GRAPHS (default 1): time loops with no host work between steps (forward
+ * propagation without snapshots, adjoint propagation without a gradient) replay a
+ * captured hipGraph of 16 steps instead of launching each kernel. Results depend on
+ * none of these settings. */
+enum fwi3d_tune_param { FWI3D_TUNE_YCHUNK = 0, FWI3D_TUNE_TX = 1, FWI3D_TUNE_GRAPHS = 2 };
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
 /* Autotuner (no counterpart in the reference): times `steps` time steps of the given
  * physics and grid on a synthetic model for every tile shape (TX 4 or 8, y chunk 4, 8,

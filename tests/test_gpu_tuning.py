@@ -70,3 +70,43 @@ def test_tune_rejects_bad_values():
             eng.tune(ychunk=-1)
     finally:
         eng.close()
+
+
+def fwd_traces(physics, nt, graphs):
+    from fwi3d_amd import workloads as W
+    shape = (21, 27, 70)
+    ny, nx, nz = shape
+    vp = np.broadcast_to(1.8 + 3.0 * np.arange(nz, dtype=np.float32) * 0.01, shape).astype(np.float32).copy()
+    rho = W.gardner(vp)
+    if physics == "acoustic":
+        model, ns, kind = W.acoustic_convert(vp, rho), 1, 0
+    elif physics == "elastic":
+        model, ns, kind = W.elastic_convert(vp, vp / 1.8, rho), 6, 1
+    else:
+        z = np.full(shape, 0.1, np.float32)
+        model, ns, kind = W.vti_convert(vp, vp / 2, rho, 0.5 * z, z, 0 * z), 6, 1
+    src = np.zeros((ns, 1, nt), np.float32)
+    src[:3 if ns == 6 else 1, 0] = W.ricker(25, nt, 0.001)
+    d = (0.01, 0.01, 0.01)
+    inj = F.Injector(kind, [[0.13, 0.1, 0.3]], shape, d)
+    ext = F.Injector(0, [[0.05 + 0.04 * i, 0.05 + 0.03 * j, 0.03] for i in range(4) for j in range(4)], shape, d)
+    eng = F.Engine(0)
+    try:
+        eng.tune(graphs=graphs)
+        par = F.make_param(nt, 0.001, sub=0, bc=(1, 2, 2, 2, 2, 2), taper=(0, 4, 4, 4, 4, 4))
+        return eng.propagate(physics, model, src, inj, ext, par, d)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("physics", ["acoustic", "elastic", "vti"])
+@pytest.mark.parametrize("nt", [20, 33, 60])
+def test_graph_replay_equals_eager(physics, nt):
+    """Forward propagation without snapshots replays a captured graph of 16 steps whose
+    kernels read the step index from a device counter (nt=20: too short, eager; 33: two
+    windows exactly; 60: three windows and an eager remainder): bit-identical traces."""
+    print(f"  {physics} nt={nt}", flush=True)
+    r_graph = fwd_traces(physics, nt, True)
+    r_eager = fwd_traces(physics, nt, False)
+    assert np.abs(r_eager).max() > 0
+    assert np.array_equal(r_graph, r_eager)
