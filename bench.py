@@ -238,7 +238,8 @@ def run_batch(args, torch, F, W):
         "data": "synthetic",
         "config": {"workload": f"batched VTI forward modelling, 32 shots over (nz,nx,ny) = (128,128,128) x11, "
                                f"(200,160,160) x11, (96,192,192) x10, nt={nt}, dt=0.0008, one launch per kernel "
-                               f"class and time step (fwi3d_batch)", "shots": len(shots)},
+                               f"class and time step, replayed from a captured graph of 16 steps between "
+                               f"departures (fwi3d_batch)", "shots": len(shots)},
         "plain_gcell_s": round(cells / t_plain / 1e9, 3),
         "shots_per_s": {"batched": round(len(shots) / t_batch, 4), "plain": round(len(shots) / t_plain, 4)},
         "speedup_vs_plain": round(t_plain / t_batch, 3),

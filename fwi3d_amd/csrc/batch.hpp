@@ -75,6 +75,11 @@ struct fwi3d_batch {
     float tune_ms[NCAND] = {0, 0, 0, 0};
     int tune_window = 0, tune_idx = -1, tune_left = 0;
     hipEvent_t tev[2] = {nullptr, nullptr};
+    // a captured graph of kGraphSteps steps of the current tables (forward-only batches),
+    // replayed over the steps between joins and departures; dropped on every rebuild
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    fwi3d::DevBuf stepctr;
 };
 
 namespace {
@@ -96,9 +101,17 @@ size_t batch_shot_bytes(int phys, const Geom& g, const fwi3d_injector* inj, cons
 }
 
 // Rebuild the device tables after shots joined or left.
+void batch_drop_graph(fwi3d_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
 void batch_rebuild(fwi3d_batch* b) {
     using B = fwi3d_batch;
     HIP_OK(hipStreamSynchronize(b->c->stream));  // the host copies below may still be in flight
+    batch_drop_graph(b);  // its replays have completed
     const int ns = (int)b->shots.size();
     b->htab.resize(ns);
     for (int s = 0; s < ns; s++) b->htab[s] = b->shots[s].d;
@@ -185,12 +198,11 @@ void batch_hooks(fwi3d_batch* b) {
 }
 
 // One global step: extraction, then per physics the time step and the injection.
-void batch_one_step(fwi3d_batch* b) {
+void batch_one_step(fwi3d_batch* b, StepIdx it) {
     using B = fwi3d_batch;
     using namespace fwi3d;
     hipStream_t st = b->c->stream;
     const BShot* T = b->tab.as<BShot>();
-    const int it = b->it;
     auto go_on = [&](hipStream_t sq, int k, auto kern, dim3 blk) {
         if (b->cls_blocks[k] > 0) hipLaunchKernelGGL(kern, dim3(b->cls_blocks[k]), blk, 0, sq, T, batch_map(b, k), it);
     };
@@ -257,8 +269,10 @@ int fwi3d_batch_destroy(fwi3d_batch* b) {
     for (auto& r : b->retired) (void)hipEventDestroy(r.second);
     for (auto& e : b->tev)
         if (e) (void)hipEventDestroy(e);
+    batch_drop_graph(b);
     b->tab.release();
     b->maps.release();
+    b->stepctr.release();
     delete b;
     return FWI3D_OK;
 }
@@ -538,8 +552,41 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
                 HIP_OK(hipEventRecord(b->tev[0], c->stream));
             }
             if (b->dirty) batch_rebuild(b);
-            batch_hooks(b);
-            batch_one_step(b);
+            // a window of steps with no host work (no gradient shot, no tuning, no departure
+            // before its last step) replays the captured graph; the last step of the window
+            // may be a departure step, handled below
+            int quiet = tuning || !c->use_graphs ? 0 : nsteps - q;
+            for (const auto& sh : b->shots) {
+                if (sh.mode != 0) quiet = 0;
+                quiet = std::min(quiet, sh.d.t0 + sh.d.nt - 1 - b->it + 1);
+            }
+            const int K = kGraphSteps;
+            if (quiet >= 2 * K) {
+                if (!b->gexec) {
+                    b->stepctr.ensure(256);
+                    int* ctr = b->stepctr.as<int>();
+                    try {
+                        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                        for (int j = 0; j < K; j++) batch_one_step(b, StepIdx(ctr, j));
+                        hipLaunchKernelGGL(advance_step, dim3(1), dim3(64), 0, c->stream, ctr, K);
+                        HIP_OK(hipStreamEndCapture(c->stream, &b->graph));
+                    } catch (...) {
+                        hipGraph_t g2 = nullptr;
+                        (void)hipStreamEndCapture(c->stream, &g2);
+                        if (g2) (void)hipGraphDestroy(g2);
+                        throw;
+                    }
+                    HIP_OK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+                }
+                const int W = quiet / K;
+                hipLaunchKernelGGL(set_step, dim3(1), dim3(64), 0, c->stream, b->stepctr.as<int>(), b->it);
+                for (int w = 0; w < W; w++) HIP_OK(hipGraphLaunch(b->gexec, c->stream));
+                b->it += W * K - 1;  // the window's last step
+                q += W * K - 1;
+            } else {
+                batch_hooks(b);
+                batch_one_step(b, StepIdx(b->it));
+            }
             // retire the shots whose last extraction (local step nt-1) was this step
             bool left = false;
             for (size_t s = 0; s < b->shots.size();) {

@@ -379,11 +379,14 @@ void srv_loop(Srv* s) {
             int nshots = 0;
             (void)fwi3d_batch_status(s->b, &nshots, nullptr, nullptr);
             if (nshots > 0) {
+                // one step per call, so that arriving requests join at the next step
                 int nact = 0;
                 if (fwi3d_batch_step(s->b, 1, &nact) != FWI3D_OK) throw Err{FWI3D_EDEVICE, s->c->err};
+                int bstep = 0;
+                (void)fwi3d_batch_status(s->b, nullptr, nullptr, &bstep);
                 {
                     std::lock_guard<std::mutex> lk(s->mu);
-                    s->steps++;
+                    s->steps = bstep;
                 }
                 int n = 0;
                 fwi3d_batch_retired(s->b, nullptr, 0, &n);
