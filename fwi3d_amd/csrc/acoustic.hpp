@@ -290,9 +290,9 @@ __global__ void acoustic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> 
                               const float* __restrict__ K, Geom g, float sdt, int branch, double* __restrict__ gK,
                               double* __restrict__ gB) {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y;
+    const int ix = blockIdx.y * blockDim.y + threadIdx.y;
     const int iy = blockIdx.z;
-    if (iz >= g.nz) return;
+    if (iz >= g.nz || ix >= g.nx) return;
     const int p = g.at(ix, iy, iz);
     const float sn = fn.scale();
     auto FN = [&](int i) { return fn.code(i); };

@@ -730,8 +730,8 @@ __global__ void elastic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> f
                              double* __restrict__ gm, double* __restrict__ gr) {
     const Geom g = A.g;
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y, iy = blockIdx.z;
-    if (iz >= g.nz) return;
+    const int ix = blockIdx.y * blockDim.y + threadIdx.y, iy = blockIdx.z;
+    if (iz >= g.nz || ix >= g.nx) return;
     const int p = g.at(ix, iy, iz);
     const int n3 = g.n;
     float F[3][3], Aj[3][3];  // [deriv axis][component]

@@ -425,13 +425,13 @@ void grad_event_acoustic(hipStream_t st, const SnapDesc& S, int itl, int nt, int
     const float sdt = (float)sub * dt;
     const int last = nt / sub;
     const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-    dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+    dim3 grid((g.nz + 63) / 64, (g.nx + 3) / 4, g.ny);  // 64 z x 4 x rows per workgroup
     with_snap_fmt(S.fmt, [&](auto tag) {
         constexpr int FMT = decltype(tag)::value;
         // a slot past the store reads as zero (the reference dereferences a null slot there)
         const SnapView<FMT> fn = snap_view<FMT>(S, itl), fm = itl > 0 ? snap_view<FMT>(S, itl - 1) : fn,
                             fp = snap_view<FMT>(S, itl + 1);
-        hipLaunchKernelGGL(acoustic_grad<FMT>, grid, dim3(64), 0, st, fm, fn, fp, adjf, M, g, sdt, branch, gK, gB);
+        hipLaunchKernelGGL(acoustic_grad<FMT>, grid, dim3(64, 4), 0, st, fm, fn, fp, adjf, M, g, sdt, branch, gK, gB);
     });
 }
 
@@ -443,16 +443,16 @@ void grad_event_elastic(hipStream_t st, bool vti, const SnapDesc& S, int itl, in
     const int last = nt / sub;
     const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
     const Geom& g = A.g;
-    dim3 grid((g.nz + 63) / 64, g.nx, g.ny);
+    dim3 grid((g.nz + 63) / 64, (g.nx + 3) / 4, g.ny);  // 64 z x 4 x rows per workgroup
     with_snap_fmt(S.fmt, [&](auto tag) {
         constexpr int FMT = decltype(tag)::value;
         const SnapView<FMT> fn = snap_view<FMT>(S, itl), fm = itl > 0 ? snap_view<FMT>(S, itl - 1) : fn,
                             fp = snap_view<FMT>(S, itl + 1);
         if (vti)
-            hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+            hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64, 4), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
                                grad, grad + nunp, grad + 2 * nunp);
         else
-            hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+            hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64, 4), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
                                grad, grad + nunp, grad + 2 * nunp);
     });
 }
