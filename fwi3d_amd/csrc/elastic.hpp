@@ -133,6 +133,7 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
         if (valid) {
             const int p = pl + col;
             const bool iny = iy >= 4 && iy < ny - 4;
+            const bool near8 = ix < 8 || ix >= nx - 8 || iy < 8 || iy >= ny - 8 || iz < 8 || iz >= nz - 8;
             float dxu[3], dyu[3], dzu[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -164,11 +165,15 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
             const float gxy = VTI ? mq[NMQ - 1] : mu;
             float* __restrict__ F = aux + nT * n3;
             F[FX_YX * n3 + p] = dxu[1] * gxy;  // G * D_x u_y  (outer D_y, component x)
-            F[FX_ZX * n3 + p] = dxu[2] * mu;   // mu * D_x u_z (outer D_z, component x)
             F[FX_XY * n3 + p] = dyu[0] * gxy;  // G * D_y u_x  (outer D_x, component y)
             F[FX_ZY * n3 + p] = dyu[2] * mu;   // mu * D_y u_z (outer D_z, component y)
-            F[FX_XZ * n3 + p] = dzu[0] * mu;   // mu * D_z u_x (outer D_x, component z)
             F[FX_YZ * n3 + p] = dzu[1] * mu;   // mu * D_z u_y (outer D_y, component z)
+            // the two in-plane fluxes are stored only where the shell and z-slab kernels
+            // read them (within 8 cells of a face); the interior kernel recomputes them
+            if (near8) {
+                F[FX_ZX * n3 + p] = dxu[2] * mu;  // mu * D_x u_z (outer D_z, component x)
+                F[FX_XZ * n3 + p] = dzu[0] * mu;  // mu * D_z u_x (outer D_x, component z)
+            }
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -195,9 +200,12 @@ __device__ __forceinline__ float d2_factor(int c, int a) {
 
 // ------------------------------------------------------------------------------------
 // Pass 2, interior points.
-// LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_zx F_xy F_zy F_xz.
+// LDS planes (x and z neighbours): u_x u_y u_z | b-fields | T_x [T_z] | F_xy F_zy.
 // Register queues (y neighbours): u (3), b-fields, T_y, F_yx, F_yz.
 // b-fields: elastic {mu}; VTI {mu, c66, c11b}.
+// The in-plane fluxes F_zx = mu D_x u_z and F_xz = mu D_z u_x are recomputed from the
+// staged u and mu planes (the u_x and u_z planes also stage the 2x2 corner cells of
+// their halos), with the flux pass's arithmetic, so they make no round trip to HBM.
 // Launched by elastic_interior_batch (batch_kernels.hpp) at 4 waves per SIMD.
 template <int TX, bool VTI, class LFn>
 __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ A, const LFn& tile_id) {
@@ -205,7 +213,7 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
     constexpr int NB = VTI ? 3 : 1;   // D2 coefficient fields
     constexpr int NT = VTI ? 3 : 1;   // normal-stress fields
     constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
-    constexpr int NPL = PF + 4;
+    constexpr int NPL = PF + 2;
     constexpr int NQ = 3 + NB + 3;
     __shared__ float s[NPL][TX + 4][TZ + 4];
     const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
@@ -250,10 +258,17 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
     for (int k = 0; k < NB; k++) psrc[PB + k] = bf[k];
     psrc[PT] = A.aux;
     if (VTI) psrc[PT + 1] = A.aux + 2 * n3;
-    psrc[PF + 0] = Ff + FX_ZX * n3;
-    psrc[PF + 1] = Ff + FX_XY * n3;
-    psrc[PF + 2] = Ff + FX_ZY * n3;
-    psrc[PF + 3] = Ff + FX_XZ * n3;
+    psrc[PF + 0] = Ff + FX_XY * n3;
+    psrc[PF + 1] = Ff + FX_ZY * n3;
+    // corner cells of the u_x and u_z planes (rows x0-2, x0-1, x0+TX, x0+TX+1 by columns
+    // z0-2, z0-1, z0+64, z0+65), loaded by the first 16 threads of the last wave
+    const int hc = (lx - (TX - 1)) * 64 + lz;
+    const bool has_c = hc >= 0 && hc < 16;
+    const int cr4 = (hc >> 2) & 3, cc4 = hc & 3;
+    const int crow = cr4 < 2 ? cr4 : TX + cr4, ccol = cc4 < 2 ? cc4 : TZ + cc4;
+    const int csrc = max(0, min(nx - 1, cr4 < 2 ? x0 - 2 + cr4 : x0 + TX + cr4 - 2)) * sx +
+                     max(0, min(nz - 1, cc4 < 2 ? z0 - 2 + cc4 : z0 + TZ + cc4 - 2));
+    float hcx = 0.f, hcz = 0.f;
 
     float q[NQ][6];  // y-2..y+2 and y+3 (loaded one iteration ahead of its use)
 #pragma unroll
@@ -266,14 +281,24 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
     float pc[NPL], hx[NPL], hz[NPL];
 #pragma unroll
     for (int k = 0; k < NPL; k++) pc[k] = hx[k] = hz[k] = 0.f;
+    // planes differentiated along x / along z (only their halos in that direction are
+    // read): u and mu along both; VTI c66 and c11b along x only (the D2 z terms use mu);
+    // the normal stress along both (elastic) or T_x along x and T_z along z (VTI); F_xy
+    // along x and F_zy along z
+    auto along_x = [](int k) { return k <= PT || k == PF + 0; };
+    auto along_z = [](int k) { return k < PB + 1 || (VTI ? k == PT + 1 : k == PT) || k == PF + 1; };
     auto prefetch = [&](int y) {
         const int pl = min(y, ny - 1) * sy;
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
             const float* __restrict__ f = psrc[k];
             if (k >= 3 + NB) pc[k] = f[pl + col];
-            if (H.has_x) hx[k] = f[pl + H.xsrc];
-            if (H.has_z) hz[k] = f[pl + H.zsrc];
+            if (along_x(k) && H.has_x) hx[k] = f[pl + H.xsrc];
+            if (along_z(k) && H.has_z) hz[k] = f[pl + H.zsrc];
+        }
+        if (has_c) {
+            hcx = curr[pl + csrc];
+            hcz = curr[2 * n3 + pl + csrc];
         }
     };
     prefetch(y0);
@@ -297,8 +322,12 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
             s[k][lx + 2][lz + 2] = (k < 3 + NB) ? q[k][2] : pc[k];
-            if (H.has_x) s[k][H.xrow][H.xcol] = hx[k];
-            if (H.has_z) s[k][H.zrow][H.zcol] = hz[k];
+            if (along_x(k) && H.has_x) s[k][H.xrow][H.xcol] = hx[k];
+            if (along_z(k) && H.has_z) s[k][H.zrow][H.zcol] = hz[k];
+        }
+        if (has_c) {
+            s[PU + 0][crow][ccol] = hcx;
+            s[PU + 2][crow][ccol] = hcz;
         }
         prefetch(iy + 1);
         __syncthreads();
@@ -309,6 +338,18 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
             auto d1x = [&](int k) { return D1C1 * (SX(k, 1) - SX(k, -1)) + D1C2 * (SX(k, 2) - SX(k, -2)); };
             auto d1z = [&](int k) { return D1C1 * (SZ(k, 1) - SZ(k, -1)) + D1C2 * (SZ(k, 2) - SZ(k, -2)); };
             auto d1y = [&](int k) { return D1C1 * (q[k][3] - q[k][1]) + D1C2 * (q[k][4] - q[k][0]); };
+            auto S2 = [&](int k, int ox, int oz) { return s[k][lx + 2 + ox][lz + 2 + oz]; };
+            // F_zx at (x, z+oz) and F_xz at (x+ox, z), as the flux pass forms them
+            auto fzx = [&](int oz) {
+                float d = D1C1 * (S2(PU + 2, 1, oz) - S2(PU + 2, -1, oz)) + D1C2 * (S2(PU + 2, 2, oz) - S2(PU + 2, -2, oz));
+                d = d / dx;
+                return d * S2(PB, 0, oz);
+            };
+            auto fxz = [&](int ox) {
+                float d = D1C1 * (S2(PU + 0, ox, 1) - S2(PU + 0, ox, -1)) + D1C2 * (S2(PU + 0, ox, 2) - S2(PU + 0, ox, -2));
+                d = d / dz;
+                return d * S2(PB, ox, 0);
+            };
             auto d2 = [&](int c, int kb, int a) -> float {
                 if (a == 0)
                     return d2_interior(SX(PU + c, -2), SX(PU + c, -1), SX(PU + c, 0), SX(PU + c, 1), SX(PU + c, 2),
@@ -329,14 +370,14 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
                 if (c == 0) {
                     acc = acc + d1x(PT) / dx;
                     acc = acc + d1y(4 + NB) / dy;     // D_y F_yx
-                    acc = acc + d1z(PF + 0) / dy;     // D_z F_zx (spacing dy as in the reference)
+                    acc = acc + (D1C1 * (fzx(1) - fzx(-1)) + D1C2 * (fzx(2) - fzx(-2))) / dy;  // D_z F_zx (spacing dy as in the reference)
                 } else if (c == 1) {
                     acc = acc + d1y(3 + NB) / dy;     // D_y T_y
-                    acc = acc + d1x(PF + 1) / dx;     // D_x F_xy
-                    acc = acc + d1z(PF + 2) / dz;     // D_z F_zy
+                    acc = acc + d1x(PF + 0) / dx;     // D_x F_xy
+                    acc = acc + d1z(PF + 1) / dz;     // D_z F_zy
                 } else {
                     acc = acc + d1z(VTI ? PT + 1 : PT) / dz;
-                    acc = acc + d1x(PF + 3) / dx;     // D_x F_xz
+                    acc = acc + (D1C1 * (fxz(1) - fxz(-1)) + D1C2 * (fxz(2) - fxz(-2))) / dx;  // D_x F_xz
                     acc = acc + d1y(5 + NB) / dy;     // D_y F_yz
                 }
                 float pv = rq[1 + c];
