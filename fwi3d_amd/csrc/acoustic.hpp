@@ -187,7 +187,10 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b,
 // Tiling of the y-marching z-slab kernel of the elastic / VTI shell (elastic.hpp): x
 // tiles from 6, y chunks from 6 to ny-6.
 struct ZSlabTiles {
-    int ntx, nyc, ychunk;
+    int ntx, nyc, ychunk;          // z slabs: x tiles from 6, y chunks from 6
+    int nzs;                       // z-slab workgroups (2 ntx nyc); the side slabs' follow
+    int sntz, snc0, snc1, schunk;  // side slabs: 32-point z tiles from 6; march chunks of
+                                   // schunk from 6 (x slabs along y, y slabs along x)
 };
 
 // ------------------------------------------------------------------------------------

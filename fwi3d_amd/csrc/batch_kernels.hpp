@@ -23,8 +23,8 @@ struct BShot {
     const float* src;
     float* rcv;
     int nshell_blk;  // shell workgroups of one component (256 points each)
-    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the y and x slabs)
-    ZSlabTiles zs;   // elastic / VTI: tiling of the z-slab kernel
+    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the edges and corners)
+    ZSlabTiles zs;   // elastic / VTI: tiling of the slab kernel (z slabs, then the side slabs)
     int ninj_blk;    // injection workgroups of one component (256 targets each)
     int next_blk;    // extraction workgroups of one component (128 receivers)
 };
@@ -138,9 +138,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void e
     elastic_boundary_block<VTI>(A, l % nb, l / nb, T[s].nshell_pts);
 }
 
-// z slabs of the shell (elastic_zslab_tile), 8 x TXZ threads per workgroup.
+// Slabs of the shell away from its edges, 256 threads per workgroup: a shot's first
+// zs.nzs workgroups tile the z slabs (elastic_zslab_tile, 8 x TXZ threads), the rest the
+// x and then the y side slabs (elastic_side_tile, 32 x 8 threads); the branch is uniform
+// per workgroup and the tiles share one LDS buffer.
 template <int TXZ, bool VTI>
 __global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
+    static_assert(8 * TXZ == 256, "z-slab and side-slab tiles both run 256 threads");
+    __shared__ float smem[slab_lds_floats<VTI>()];
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
     const int k = it.value() - T[s].t0;
@@ -148,7 +153,14 @@ __global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __re
     EArgs A = T[s].ef;
     bshot_rotate(A, T[s], k);
     const int l = L - m.first(s);
-    elastic_zslab_tile<TXZ, VTI>(A, T[s].zs, [&] { return l; });
+    const ZSlabTiles Z = T[s].zs;
+    if (l < Z.nzs) {
+        elastic_zslab_tile<TXZ, VTI>(A, Z, [&] { return l; }, smem);
+    } else {
+        const int r = l - Z.nzs, n0 = 2 * Z.sntz * Z.snc0;
+        if (r < n0) elastic_side_tile<0, VTI>(A, Z, r, smem);
+        else elastic_side_tile<1, VTI>(A, Z, r - n0, smem);
+    }
 }
 
 // ---- source injection into `next` after the step (mode 0), receivers before it.

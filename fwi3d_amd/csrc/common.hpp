@@ -297,6 +297,44 @@ __device__ __forceinline__ bool shell_point(int t, int w, int nx, int ny, int nz
     return false;
 }
 
+// Edges and corners of the shell: the points within w cells of faces of two or three
+// different axes (what the slab kernels leave to the point-wise shell kernel).
+// Enumerated as: in each of the 2w y-planes at the ends, the 2w x-rows (all z), then the
+// 2w z-points of the columns between them; then, for the y planes between, the 2w z-points
+// of the 2w x-rows. Returns false past the end.
+__host__ __device__ inline int edge_count(int w, int nx, int ny, int nz) {
+    return 2 * w * (2 * w * nz + (nx - 2 * w) * 2 * w) + (ny - 2 * w) * 4 * w * w;
+}
+__device__ __forceinline__ bool edge_point(int t, int w, int nx, int ny, int nz, int& ix, int& iy, int& iz) {
+    const int per = 2 * w * nz + (nx - 2 * w) * 2 * w;
+    if (t < 2 * w * per) {
+        const int pl = t / per;
+        int r = t - pl * per;
+        iy = pl < w ? pl : ny - 2 * w + pl;
+        if (r < 2 * w * nz) {
+            const int xr = r / nz;
+            ix = xr < w ? xr : nx - 2 * w + xr;
+            iz = r - xr * nz;
+        } else {
+            r -= 2 * w * nz;
+            const int xi = r / (2 * w), zr = r - xi * (2 * w);
+            ix = w + xi;
+            iz = zr < w ? zr : nz - 2 * w + zr;
+        }
+        return true;
+    }
+    t -= 2 * w * per;
+    const int per2 = 4 * w * w;
+    const int pl = t / per2;
+    if (pl >= ny - 2 * w) return false;
+    const int r = t - pl * per2;
+    const int xr = r / (2 * w), zr = r - xr * (2 * w);
+    iy = w + pl;
+    ix = xr < w ? xr : nx - 2 * w + xr;
+    iz = zr < w ? zr : nz - 2 * w + zr;
+    return true;
+}
+
 __host__ __device__ inline int shell_count(int w, int nx, int ny, int nz) {
     return 2 * w * nx * nz + (ny - 2 * w) * 2 * w * nz + (ny - 2 * w) * (nx - 2 * w) * 2 * w;
 }

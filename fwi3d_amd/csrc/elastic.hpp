@@ -531,13 +531,14 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
     }
 }
 
-// Shell workgroup b (256 points of the shell enumeration), displacement component c;
-// points from npts on (the z slabs, which elastic_zslab_tile computes) are skipped.
+// Shell workgroup b (256 points of the edge enumeration, edge_point), displacement
+// component c. The slabs away from the edges are tiled by elastic_zslab_tile and
+// elastic_side_tile.
 template <bool VTI>
 __device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, int c, int npts) {
     int ix, iy, iz;
     const int t = b * 256 + threadIdx.x;
-    if (t >= npts || !shell_point(t, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
+    if (t >= npts || !edge_point(t, 6, A.g.nx, A.g.ny, A.g.nz, ix, iy, iz)) return;
     if (c == 0) elastic_boundary_point<VTI, 0>(A, ix, iy, iz);
     else if (c == 1) elastic_boundary_point<VTI, 1>(A, ix, iy, iz);
     else elastic_boundary_point<VTI, 2>(A, ix, iy, iz);
@@ -555,14 +556,21 @@ __device__ __forceinline__ void elastic_boundary_block(const EArgs& A, int b, in
 // Tiling: ZSlabTiles (acoustic.hpp; the x tiles start at 6, the y chunks at 6 and stop
 // at ny-6).
 
+// LDS of the slab kernels (z-slab and side-slab tiles share it): [planes][36][8] or
+// [planes][8][36] floats.
+template <bool VTI>
+constexpr int slab_lds_floats() { return ((VTI ? 3 : 1) + 3 + (VTI ? 2 : 1) + 4) * 36 * 8; }
+
 template <int TXZ, bool VTI, class LFn>
-__device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, const ZSlabTiles Z, const LFn& tile_id) {
+__device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, const ZSlabTiles Z, const LFn& tile_id,
+                                                   float* smem) {
+    static_assert(TXZ == 32, "the slab kernels share a [planes][36][8] LDS buffer");
     constexpr int NB = VTI ? 3 : 1;
     constexpr int NT = VTI ? 3 : 1;
     constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
     constexpr int NPL = PF + 4;
     constexpr int NQ = 3 + NB + 3;
-    __shared__ float s[NPL][TXZ + 4][8];
+    float(*s)[TXZ + 4][8] = reinterpret_cast<float(*)[TXZ + 4][8]>(smem);
     const Geom& g = A.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz, sx = g.sx, sy = g.sy, n3 = g.n;
     const float* __restrict__ curr = A.curr;
@@ -725,6 +733,222 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
 
 // Source injection, all three displacement components (blockIdx.y):
 // next += taper(dt^2 inj / rho / scale); mode 1: initial prev = 0.5 dt^2 inj(src[0]) / rho.
+// ------------------------------------------------------------------------------------
+// Pass 2, side slabs: the points within 6 cells of an x face (AX = 0) or a y face
+// (AX = 1) and at least 6 cells inside the faces of the two other axes. A tile is 32 z
+// lanes x the 8 points nearest the face along its normal axis (the slab's 6 rows plus
+// the 2 further rows its closure rows read; a half-wave per row), marching along the
+// remaining in-plane axis (y for x slabs, x for y slabs) with register queues: the
+// normal-axis closure rows, the face's traction SAT and the z stencils read the staged
+// x-z or y-z planes in LDS. Arithmetic and order of operations are those of
+// elastic_boundary_point. Tiling (ZSlabTiles): z tiles of 32 from z = 6, march chunks of
+// `schunk` from 6 to n - 6; tile L: bit 0 the low or high slab, then z tile, then chunk.
+template <int AX, bool VTI>
+__device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, const ZSlabTiles Z, int L, float* smem) {
+    constexpr int NB = VTI ? 3 : 1;
+    constexpr int NT = VTI ? 3 : 1;
+    constexpr int MX = 1 - AX;  // marching axis
+    constexpr int PU = 0, PB = 3, PT = 3 + NB, PF = PT + (VTI ? 2 : 1);
+    constexpr int NPL = PF + 4;
+    constexpr int NQ = 3 + NB + 3;
+    float(*s)[8][36] = reinterpret_cast<float(*)[8][36]>(smem);
+    const Geom& g = A.g;
+    const int nz = g.nz, n3 = g.n;
+    const int sA = AX == 0 ? g.sx : g.sy, sM = AX == 0 ? g.sy : g.sx;
+    const int nA = AX == 0 ? g.nx : g.ny, nM = AX == 0 ? g.ny : g.nx;
+    const float hA = AX == 0 ? g.dx : g.dy, hM = AX == 0 ? g.dy : g.dx;
+    const float* __restrict__ curr = A.curr;
+    const float* __restrict__ prev = A.prev;
+    const float* bf[3] = {A.mu, VTI ? A.c66 : A.mu, VTI ? A.c11b : A.mu};
+    const float* Ff = A.aux + NT * n3;
+    const int slab = L & 1, tz = (L >> 1) % Z.sntz, ch = (L >> 1) / Z.sntz;
+    const bool hi = slab != 0;
+    const int lz = threadIdx.x & 31, lr = threadIdx.x >> 5;  // z lane, LDS row
+    const int z0 = 6 + tz * 32, iz = z0 + lz;
+    const int base = hi ? nA - 8 : 0;  // normal-axis index of LDS row 0
+    const int ia = base + lr;
+    const int r = hi ? 7 - lr : lr;  // row counted from the face
+    const bool act = iz < nz - 6 && r < 6;
+    const bool face = r == 0;
+    const int m0 = 6 + ch * Z.schunk, m1 = min(nM - 6, m0 + Z.schunk);
+    const int col = ia * sA + min(iz, nz - 1);
+    auto ldm = [&](const float* __restrict__ f, int m) { return f[max(0, min(nM - 1, m)) * sM + col]; };
+    // z-halo value of this thread: threads 0..31 load rows 0..7 at z0-2, z0-1, z0+32, z0+33
+    const int h = threadIdx.x;
+    const bool has_h = h < 32;
+    const int hrow = (h >> 2) & 7, hz4 = h & 3;
+    const int hcol = hz4 < 2 ? hz4 : 32 + hz4;
+    const int hsrc = (base + hrow) * sA + max(0, min(nz - 1, hz4 < 2 ? z0 - 2 + hz4 : z0 + 32 + hz4 - 2));
+    // threads without a halo value reload their own point (a cache hit), which keeps the
+    // loads unconditional
+    const int hoff = has_h ? hsrc : col;
+
+    // queues (derivatives along the march axis): u, b-fields, then the normal stress and
+    // the two fluxes whose outer derivative runs along it
+    const float* qsrc[NQ];
+#pragma unroll
+    for (int c = 0; c < 3; c++) qsrc[c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) qsrc[3 + k] = bf[k];
+    // planes (derivatives along the normal axis and z): u, b-fields, the normal stress
+    // of the normal axis [and T_z], then F_zx, the flux with outer derivative along the
+    // normal axis for component x or y, F_zy, the one for component z
+    const float* psrc[NPL];
+#pragma unroll
+    for (int c = 0; c < 3; c++) psrc[PU + c] = curr + c * n3;
+#pragma unroll
+    for (int k = 0; k < NB; k++) psrc[PB + k] = bf[k];
+    if (AX == 0) {
+        qsrc[3 + NB] = A.aux + (VTI ? n3 : 0);  // T_y
+        qsrc[4 + NB] = Ff + FX_YX * n3;
+        qsrc[5 + NB] = Ff + FX_YZ * n3;
+        psrc[PT] = A.aux;                        // T_x
+        psrc[PF + 1] = Ff + FX_XY * n3;
+        psrc[PF + 3] = Ff + FX_XZ * n3;
+    } else {
+        qsrc[3 + NB] = A.aux;                    // T_x
+        qsrc[4 + NB] = Ff + FX_XY * n3;
+        qsrc[5 + NB] = Ff + FX_XZ * n3;
+        psrc[PT] = A.aux + (VTI ? n3 : 0);       // T_y
+        psrc[PF + 1] = Ff + FX_YX * n3;
+        psrc[PF + 3] = Ff + FX_YZ * n3;
+    }
+    if (VTI) psrc[PT + 1] = A.aux + 2 * n3;      // T_z
+    psrc[PF + 0] = Ff + FX_ZX * n3;
+    psrc[PF + 2] = Ff + FX_ZY * n3;
+
+    float q[NQ][6];
+#pragma unroll
+    for (int k = 0; k < NQ; k++)
+#pragma unroll
+        for (int o = 0; o < 5; o++) q[k][o] = ldm(qsrc[k], m0 - 2 + o);
+    float pc[NPL], hv[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; k++) pc[k] = hv[k] = 0.f;
+    auto prefetch = [&](int m) {
+        const int pl = min(m, nM - 1) * sM;
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            if (k >= 3 + NB) pc[k] = psrc[k][pl + col];
+            hv[k] = psrc[k][pl + hoff];
+        }
+    };
+    prefetch(m0);
+    const ECoef<VTI> M{A.lam, A.mu, A.rho, A.c13, A.eps, A.c66};
+    const int bcf = AX == 0 ? (hi ? A.bc[3] : A.bc[2]) : (hi ? A.bc[5] : A.bc[4]);  // constant indices
+
+    for (int im = m0; im < m1; im++) {
+#pragma unroll
+        for (int k = 0; k < NQ; k++) q[k][5] = ldm(qsrc[k], im + 3);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPL; k++) {
+            s[k][lr][lz + 2] = (k < 3 + NB) ? q[k][2] : pc[k];
+            if (has_h) s[k][hrow][hcol] = hv[k];
+        }
+        prefetch(im + 1);
+        __syncthreads();
+        if (act) {
+            const int p = im * sM + col;
+            const int ix = AX == 0 ? ia : im, iy = AX == 0 ? im : ia;
+            auto FN = [&](int k, int j) { return s[k][hi ? 7 - j : j][lz + 2]; };  // j-th point from the face
+            auto dA = [&](int k) {  // along the normal axis (times h)
+                if (r >= 4)
+                    return D1C1 * (s[k][lr + 1][lz + 2] - s[k][lr - 1][lz + 2]) +
+                           D1C2 * (s[k][lr + 2][lz + 2] - s[k][lr - 2][lz + 2]);
+                return d1_closure_at(!hi, r, [&](int j) { return FN(k, j); });
+            };
+            auto dZ = [&](int k) {
+                return D1C1 * (s[k][lr][lz + 3] - s[k][lr][lz + 1]) + D1C2 * (s[k][lr][lz + 4] - s[k][lr][lz]);
+            };
+            auto dM = [&](int k) { return D1C1 * (q[k][3] - q[k][1]) + D1C2 * (q[k][4] - q[k][0]); };
+            auto d2 = [&](int c, int kb, int a) -> float {
+                if (a == AX) return d2_rows(r, [&](int j) { return FN(PU + c, j); }, [&](int k) { return FN(PB + kb, k); });
+                if (a == MX)
+                    return d2_interior(q[c][0], q[c][1], q[c][2], q[c][3], q[c][4], q[3 + kb][0], q[3 + kb][1],
+                                       q[3 + kb][2], q[3 + kb][3], q[3 + kb][4]);
+                return d2_interior(s[PU + c][lr][lz], s[PU + c][lr][lz + 1], s[PU + c][lr][lz + 2], s[PU + c][lr][lz + 3],
+                                   s[PU + c][lr][lz + 4], s[PB + kb][lr][lz], s[PB + kb][lr][lz + 1],
+                                   s[PB + kb][lr][lz + 2], s[PB + kb][lr][lz + 3], s[PB + kb][lr][lz + 4]);
+            };
+            auto sats = [&](int k) {  // outward S (times h) at the face point
+                return c_sat_s[0] * FN(k, 0) + c_sat_s[1] * FN(k, 1) + c_sat_s[2] * FN(k, 2) + c_sat_s[3] * FN(k, 3);
+            };
+            const float rho = A.rho[p];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2(c, d2_bfield<VTI>(c, 0), 0);
+                acc = acc + (d2_factor<VTI>(c, 1) / (g.dy * g.dy)) * d2(c, d2_bfield<VTI>(c, 1), 1);
+                acc = acc + (d2_factor<VTI>(c, 2) / (g.dz * g.dz)) * d2(c, d2_bfield<VTI>(c, 2), 2);
+                if (AX == 0) {
+                    if (c == 0) {
+                        acc = acc + dA(PT) / g.dx;          // D_x T_x
+                        acc = acc + dM(4 + NB) / g.dy;      // D_y F_yx
+                        acc = acc + dZ(PF + 0) / g.dy;      // D_z F_zx (spacing dy as in the reference)
+                    } else if (c == 1) {
+                        acc = acc + dM(3 + NB) / g.dy;      // D_y T_y
+                        acc = acc + dA(PF + 1) / g.dx;      // D_x F_xy
+                        acc = acc + dZ(PF + 2) / g.dz;      // D_z F_zy
+                    } else {
+                        acc = acc + dZ(VTI ? PT + 1 : PT) / g.dz;
+                        acc = acc + dA(PF + 3) / g.dx;      // D_x F_xz
+                        acc = acc + dM(5 + NB) / g.dy;      // D_y F_yz
+                    }
+                } else {
+                    if (c == 0) {
+                        acc = acc + dM(3 + NB) / g.dx;      // D_x T_x
+                        acc = acc + dA(PF + 1) / g.dy;      // D_y F_yx
+                        acc = acc + dZ(PF + 0) / g.dy;      // D_z F_zx (spacing dy as in the reference)
+                    } else if (c == 1) {
+                        acc = acc + dA(PT) / g.dy;          // D_y T_y
+                        acc = acc + dM(4 + NB) / g.dx;      // D_x F_xy
+                        acc = acc + dZ(PF + 2) / g.dz;      // D_z F_zy
+                    } else {
+                        acc = acc + dZ(VTI ? PT + 1 : PT) / g.dz;
+                        acc = acc + dM(5 + NB) / g.dx;      // D_x F_xz
+                        acc = acc + dA(PF + 3) / g.dy;      // D_y F_yz
+                    }
+                }
+                float pv = prev[c * n3 + p];
+                if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+                if (face && (bcf == 1 || bcf == 2)) {
+                    // traction SAT of this face (elastic_boundary_point, normal axis AX;
+                    // tangential axes: the march axis and z)
+                    const float sg = hi ? 1.f : -1.f;
+                    const float adh = 1.0f / (hA * SAT_H0);
+                    float term;
+                    if (c == AX) {
+                        const float tang = M.cnorm(AX, MX, p) * (dM(MX) / hM) + M.cnorm(AX, 2, p) * (dZ(PU + 2) / g.dz) +
+                                           M.cnorm(AX, AX, p) * (dA(PU + AX) / hA);
+                        term = sg * tang + M.knn(AX, p) * sats(PU + AX) / hA;
+                        if (bcf == 2) term -= M.pimp(AX, p) * pv / A.dt;
+                    } else {
+                        const float G = M.gshear(AX, c, p);
+                        const float du = c == 2 ? dZ(PU + AX) / g.dz : dM(AX) / hM;  // D_c u_AX
+                        term = sg * G * du + G * sats(PU + c) / hA;
+                        if (bcf == 2) term -= 0.5f * sqrtf(M.mu[p] * M.rho[p]) * pv / A.dt;
+                    }
+                    acc -= adh * term;
+                }
+                float nv = A.dt2 * acc / rho + 2.f * q[c][2] - pv;
+                if (face && bcf == 2) {
+                    // e_scale of an x- or y-face point away from the other faces: only this
+                    // face absorbs (normal component: the P speed, (1 + 2 eps)-scaled for
+                    // VTI; tangential: the S speed)
+                    const float F = VTI ? 1.f + 2.f * M.eps[p] : 1.f;
+                    const float v = c == AX ? sqrtf(F * (M.lam[p] + 2.f * M.mu[p]) / M.rho[p]) : sqrtf(M.mu[p] / M.rho[p]);
+                    nv = nv / (1.f + (0.f + v * A.dt / (2.f * hA * SAT_H0)));
+                }
+                A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            q[k][0] = q[k][1]; q[k][1] = q[k][2]; q[k][2] = q[k][3]; q[k][3] = q[k][4]; q[k][4] = q[k][5];
+        }
+    }
+}
+
 template <bool VTI>
 __device__ __forceinline__ void elastic_inject_one(float* __restrict__ out3, const EArgs& A, const InjTab& I,
                                                    const float* __restrict__ src, int nt, int it, int mode, int j, int comp) {

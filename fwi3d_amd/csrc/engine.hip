@@ -316,19 +316,28 @@ int tile_grid_yc(Args& A, const Geom& g, int TX, bool interior, int ychunk) {
     A.nyc = (ey + A.ychunk - 1) / A.ychunk;
     return A.ntz * A.ntx * A.nyc;  // 0 when the interior is empty (an axis of 12 points)
 }
-// z-slab kernel of the elastic / VTI shell: x rows per workgroup and y planes per chunk.
+// Slab kernel of the elastic / VTI shell. z slabs: x rows per workgroup and y planes per
+// chunk; side (x and y) slabs: march planes per chunk.
 constexpr int kZSlabTX = 32;
 constexpr int kZSlabYChunk = 3;
-// Its tiling (0 workgroups when the slabs have no points inside the x / y slabs).
+constexpr int kSideChunk = 3;
+// Its tiling: the z slabs inside the x and y slabs, then the x slabs and the y slabs
+// inside the other two axes' slabs (0 workgroups where an axis has no points 6 cells
+// inside its faces).
 int zslab_tiles(ZSlabTiles& Z, const Geom& g) {
     Z.ychunk = kZSlabYChunk;
     Z.ntx = (g.nx - 12 + kZSlabTX - 1) / kZSlabTX;
     Z.nyc = (g.ny - 12 + Z.ychunk - 1) / Z.ychunk;
-    return 2 * Z.ntx * Z.nyc;
+    Z.nzs = 2 * Z.ntx * Z.nyc;
+    Z.schunk = kSideChunk;
+    Z.sntz = (g.nz - 12 + 31) / 32;
+    Z.snc0 = (g.ny - 12 + kSideChunk - 1) / kSideChunk;
+    Z.snc1 = (g.nx - 12 + kSideChunk - 1) / kSideChunk;
+    return Z.nzs + 2 * Z.sntz * (Z.snc0 + Z.snc1);
 }
-// Shell points left to the point-wise shell kernel once the z slabs are tiled: the
-// y slabs, then the x slabs (the first part of shell_point's enumeration).
-int shell_xy_count(const Geom& g) { return shell_count(6, g.nx, g.ny, g.nz) - (g.ny - 12) * (g.nx - 12) * 12; }
+// Shell points left to the point-wise shell kernel once the slabs are tiled: the edges
+// and corners (edge_point's enumeration).
+int shell_xy_count(const Geom& g) { return edge_count(6, g.nx, g.ny, g.nz); }
 
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
 // grid on the context, else the context's settings (fwi3d_tune).
