@@ -440,7 +440,7 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             const int ni = tile_grid_yc(d.ei, g, TX, true, yc);
             s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = nf;
             s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = ni;
-            d.nshell_pts = shell_xy_count(g);
+            d.nshell_pts = shell_edge_count(g);
             d.nshell_blk = (d.nshell_pts + 255) / 256;
             s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 3 * d.nshell_blk;
             s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g);

@@ -337,7 +337,7 @@ int zslab_tiles(ZSlabTiles& Z, const Geom& g) {
 }
 // Shell points left to the point-wise shell kernel once the slabs are tiled: the edges
 // and corners (edge_point's enumeration).
-int shell_xy_count(const Geom& g) { return edge_count(6, g.nx, g.ny, g.nz); }
+int shell_edge_count(const Geom& g) { return edge_count(6, g.nx, g.ny, g.nz); }
 
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
 // grid on the context, else the context's settings (fwi3d_tune).
@@ -645,13 +645,11 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int nblk = tile_grid_yc(A, g, TX, false, ychunk);
     EArgs AI = A;
     const int nint = tile_grid_yc(AI, g, TX, true, ychunk);
-    const int nshell = shell_count(6, g.nx, g.ny, g.nz);  // points within 6 cells of a face
     // one-shot table for the time-step kernels (rotation from the step index)
     BShot T1h{};
     T1h.phys = vti ? FWI3D_VTI : FWI3D_ELASTIC; T1h.t0 = 0; T1h.nt = par->nt; T1h.nf = 3 * n; T1h.F = F; T1h.g = g;
     T1h.ef = A; T1h.ei = AI;
-    (void)nshell;
-    T1h.nshell_pts = shell_xy_count(g);
+    T1h.nshell_pts = shell_edge_count(g);
     T1h.nshell_blk = (T1h.nshell_pts + 255) / 256;
     const int nzslab = zslab_tiles(T1h.zs, g);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
