@@ -184,8 +184,8 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b,
     A.next[p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
 }
 
-// Tiling of the y-marching z-slab kernel of the elastic / VTI shell (elastic.hpp): x
-// tiles from 6, y chunks from 6 to ny-6.
+// Tiling of the slab kernel of the elastic / VTI shell (elastic.hpp): the y-marching
+// z-slab tiles, then the side-slab tiles of the x and y slabs.
 struct ZSlabTiles {
     int ntx, nyc, ychunk;          // z slabs: x tiles from 6, y chunks from 6
     int nzs;                       // z-slab workgroups (2 ntx nyc); the side slabs' follow

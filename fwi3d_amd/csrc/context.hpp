@@ -76,7 +76,7 @@ struct fwi3d_ctx {
     bool snap_valid = false;
     int snap_dims[3] = {0, 0, 0};
     // profiling: [0] interior time-step kernel, [1] flux pre-pass (elastic/VTI),
-    // [2] boundary-shell time-step kernel, [3] z-slab kernel (elastic/VTI), [4] the whole
+    // [2] boundary-shell time-step kernel, [3] slab kernel (elastic/VTI), [4] the whole
     // time step (its kernels, launched one after another)
     // tiling of the y-marching kernels (fwi3d_tune)
     int tune_ychunk = 0;  // y planes per workgroup (0: default 8)

@@ -1,16 +1,18 @@
 // Elastic isotropic and VTI kernels (gfx950): one time step of
 // nl_we_op_e::propagate (src/cpp/we_op.cpp:646-836) / nl_we_op_vti::propagate
-// (we_op.cpp:1290-1497) in three launches:
+// (we_op.cpp:1290-1497) in four launches (batch_kernels.hpp):
 //
-//   elastic_flux      (whole grid) first derivatives of the displacement -> normal
-//                     stress T (lambda*div u; VTI: three c11c/c12/c13 combinations) and
-//                     the six shear fluxes F_ac = G * D_c u_a (a != c; G = mu, or c66
-//                     for the x-y coupling in VTI), written to HBM.
-//   elastic_interior  points 6+ cells inside every face: 9x D2(b) u_c + D_c T_c +
-//                     D_a F_ac from LDS x-z planes and y register queues, leapfrog
-//                     update, cos^2 taper.
-//   elastic_boundary  the 6-cell shell: the same terms with SBP closure rows, the
-//                     traction SATs, the update, the absorbing scaling and the taper.
+//   flux      (whole grid) first derivatives of the displacement -> normal stress T
+//             (lambda*div u; VTI: three c11c/c12/c13 combinations) and the six shear
+//             fluxes F_ac = G * D_c u_a (a != c; G = mu, or c66 for the x-y coupling in
+//             VTI), written to HBM (F_zx and F_xz only near the faces).
+//   interior  points 6+ cells inside every face: 9x D2(b) u_c + D_c T_c + D_a F_ac from
+//             LDS x-z planes and y register queues, leapfrog update, cos^2 taper.
+//   boundary  the edges and corners of the 6-cell shell, point-wise: the same terms
+//             with SBP closure rows, the traction SATs, the update, the absorbing
+//             scaling and the taper.
+//   slabs     the rest of the shell, marching: the z slabs (elastic_zslab_tile) and
+//             the x and y slabs (elastic_side_tile).
 //
 // The displacement components are stored one after the other ([3][y][x][z]).
 // Flux layout in the aux buffer: [T (1 or 3)][F_yx F_zx F_xy F_zy F_xz F_yz].
@@ -168,7 +170,7 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
             F[FX_XY * n3 + p] = dyu[0] * gxy;  // G * D_y u_x  (outer D_x, component y)
             F[FX_ZY * n3 + p] = dyu[2] * mu;   // mu * D_y u_z (outer D_z, component y)
             F[FX_YZ * n3 + p] = dzu[1] * mu;   // mu * D_z u_y (outer D_y, component z)
-            // the two in-plane fluxes are stored only where the shell and z-slab kernels
+            // the two in-plane fluxes are stored only where the shell and slab kernels
             // read them (within 8 cells of a face); the interior kernel recomputes them
             if (near8) {
                 F[FX_ZX * n3 + p] = dxu[2] * mu;  // mu * D_x u_z (outer D_z, component x)

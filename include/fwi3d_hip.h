@@ -101,8 +101,9 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
  * work), 0 stops.
  * read() returns the summed milliseconds and launch count since the last enable of
  * kernel class `kernel`: 0 = the interior time-step kernel, 1 = the elastic/VTI flux
- * pre-pass, 2 = the boundary-shell time-step kernel, 3 = the elastic/VTI z-slab kernel
- * (the top and bottom 6 planes away from the x and y faces), 4 = the whole time step. */
+ * pre-pass, 2 = the boundary-shell time-step kernel (elastic/VTI: the shell's edges and
+ * corners), 3 = the elastic/VTI slab kernel (the z slabs away from the x and y faces and
+ * the x and y slabs away from the other faces), 4 = the whole time step. */
 /* Tiling of the y-marching time-step kernels, per context (no counterpart in the
  * reference; the tile shapes an autotuner picks): FWI3D_TUNE_YCHUNK = y planes per
  * workgroup (0: default 8), FWI3D_TUNE_TX = x rows per workgroup (4 or 8).
