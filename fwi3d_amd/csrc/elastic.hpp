@@ -991,27 +991,64 @@ __global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, cons
 // (we_op.cpp:488-565, 1108-1216) with FP64 accumulation. `lag` is the adjoint field
 // of the previous step (the reference uses the derivatives computed one step earlier);
 // `adj` the current adjoint field (rho term). Gradients in the unpadded layout.
+// Workgroups of 64 (z) x 4 (x) points of one y plane: the forward-field codes and the
+// lagged adjoint field of the tile and its 2-cell x / z halos are staged in LDS once, so
+// the interior x and z derivatives read on-chip values (closure rows and the y
+// derivatives read memory, as d1_acc / d1_point do; same arithmetic).
 template <bool VTI, int FMT>
-__global__ void elastic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> fp, const float* __restrict__ adj,
-                             const float* __restrict__ lag, EArgs A, float sdt, int branch, double* __restrict__ gl,
-                             double* __restrict__ gm, double* __restrict__ gr) {
+__global__ __launch_bounds__(256) void elastic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> fp,
+                                                    const float* __restrict__ adj, const float* __restrict__ lag, EArgs A,
+                                                    float sdt, int branch, double* __restrict__ gl, double* __restrict__ gm,
+                                                    double* __restrict__ gr) {
+    __shared__ float sf[3][8][68], sl[3][8][68];  // [component][x row x0-2..x0+5][z column z0-2..z0+65]
     const Geom g = A.g;
-    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = blockIdx.y * blockDim.y + threadIdx.y, iy = blockIdx.z;
-    if (iz >= g.nz || ix >= g.nx) return;
-    const int p = g.at(ix, iy, iz);
+    const int lz = threadIdx.x, lx = threadIdx.y;
+    const int z0 = blockIdx.x * 64, x0 = blockIdx.y * 4, iy = blockIdx.z;
+    const int iz = z0 + lz, ix = x0 + lx;
     const int n3 = g.n;
+    const int pl = iy * g.sy;
+    const int p = pl + min(ix, g.nx - 1) * g.sx + min(iz, g.nz - 1);
+    HaloMap<4> H;
+    H.init(lx, lz, x0, z0, g.nx, g.nz, g.sx);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        sf[c][lx + 2][lz + 2] = fn.code(c * n3 + p);
+        sl[c][lx + 2][lz + 2] = lag[c * n3 + p];
+        if (H.has_x) {
+            sf[c][H.xrow][H.xcol] = fn.code(c * n3 + pl + H.xsrc);
+            sl[c][H.xrow][H.xcol] = lag[c * n3 + pl + H.xsrc];
+        }
+        if (H.has_z) {
+            sf[c][H.zrow][H.zcol] = fn.code(c * n3 + pl + H.zsrc);
+            sl[c][H.zrow][H.zcol] = lag[c * n3 + pl + H.zsrc];
+        }
+    }
+    __syncthreads();
+    if (iz >= g.nz || ix >= g.nx) return;
+    const bool inx = ix >= 4 && ix < g.nx - 4, inz = iz >= 4 && iz < g.nz - 4;
     float F[3][3], Aj[3][3];  // [deriv axis][component]
     const float sn = fn.scale();
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         auto FN = [&](int i) { return fn.code(c * n3 + i); };
-        F[0][c] = d1_acc(FN, p, ix, g.nx, g.sx) * sn / g.dx;
+        const float fx = inx ? D1C1 * (sf[c][lx + 3][lz + 2] - sf[c][lx + 1][lz + 2]) +
+                                   D1C2 * (sf[c][lx + 4][lz + 2] - sf[c][lx][lz + 2])
+                             : d1_acc(FN, p, ix, g.nx, g.sx);
+        const float fz = inz ? D1C1 * (sf[c][lx + 2][lz + 3] - sf[c][lx + 2][lz + 1]) +
+                                   D1C2 * (sf[c][lx + 2][lz + 4] - sf[c][lx + 2][lz])
+                             : d1_acc(FN, p, iz, g.nz, 1);
+        const float ax = inx ? D1C1 * (sl[c][lx + 3][lz + 2] - sl[c][lx + 1][lz + 2]) +
+                                   D1C2 * (sl[c][lx + 4][lz + 2] - sl[c][lx][lz + 2])
+                             : d1_point(lag + c * n3, p, ix, g.nx, g.sx);
+        const float az = inz ? D1C1 * (sl[c][lx + 2][lz + 3] - sl[c][lx + 2][lz + 1]) +
+                                   D1C2 * (sl[c][lx + 2][lz + 4] - sl[c][lx + 2][lz])
+                             : d1_point(lag + c * n3, p, iz, g.nz, 1);
+        F[0][c] = fx * sn / g.dx;
         F[1][c] = d1_acc(FN, p, iy, g.ny, g.sy) * sn / g.dy;
-        F[2][c] = d1_acc(FN, p, iz, g.nz, 1) * sn / g.dz;
-        Aj[0][c] = d1_point(lag + c * n3, p, ix, g.nx, g.sx) / g.dx;
+        F[2][c] = fz * sn / g.dz;
+        Aj[0][c] = ax / g.dx;
         Aj[1][c] = d1_point(lag + c * n3, p, iy, g.ny, g.sy) / g.dy;
-        Aj[2][c] = d1_point(lag + c * n3, p, iz, g.nz, 1) / g.dz;
+        Aj[2][c] = az / g.dz;
     }
     // forward-field terms named as in the reference: temp[0..8]
     const float t0 = F[0][0], t1 = F[1][1], t2 = F[2][2], t3 = F[0][2], t4 = F[2][0], t5 = F[0][1], t6 = F[1][0],
