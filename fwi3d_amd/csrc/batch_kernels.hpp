@@ -92,10 +92,10 @@ __global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __re
     acoustic_boundary_block(A, L - m.first(s), T[s].nshell_pts);
 }
 
-// Occupancy: 5 waves per SIMD (96 VGPRs, 8 B/lane spilled) measured faster than the
-// compiler's 4 waves without spills.
+// Occupancy: elastic 5 waves per SIMD (96 VGPRs, 12 B/lane spilled) measured faster than
+// the compiler's 4 waves without spills; VTI (more model fields per point) is faster at 4.
 template <int TX, bool VTI>
-__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(5))) void elastic_flux_batch(
+__global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(VTI ? 4 : 5))) void elastic_flux_batch(
     const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = m.shot_of(L);
@@ -122,11 +122,11 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) vo
     elastic_interior_tile<TX, VTI>(A, [&] { return l; });
 }
 
-// Shell workgroups are not XCD-remapped: dispatch order gives every XCD the same mix of
-// the cheap y/x slabs and the costlier z slabs. Occupancy: 6 waves per SIMD (80 VGPRs,
-// 20-40 B/lane spilled) measured faster than 5.
+// Edges and corners of the shell, point-wise. Workgroups are not XCD-remapped: dispatch
+// order gives every XCD the same mix of the rows along x and the z columns.
+// Occupancy: 5 waves per SIMD measured faster than 6 and 7.
 template <bool VTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void elastic_boundary_batch(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void elastic_boundary_batch(
     const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
     const int s = m.shot_of(L);
