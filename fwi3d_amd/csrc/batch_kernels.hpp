@@ -92,6 +92,32 @@ __global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __re
     acoustic_boundary_block(A, L - m.first(s), T[s].nshell_pts);
 }
 
+// Receivers of the single shot T[0] at local step k (they read curr, which no kernel of
+// the step writes), 256 per workgroup, component-major.
+__device__ __forceinline__ void extract_block256(const BShot& S, int l, int k) {
+    const int nb = (S.E.npts + 255) / 256;
+    const int r = (l % nb) * 256 + threadIdx.x;
+    if (r >= S.E.npts) return;
+    const float* curr = S.F + (long long)((k + 1) % 3) * S.nf;
+    extract_one(curr, S.g, S.E, S.rcv, S.nt, k, r, l / nb);
+}
+__host__ __device__ inline int extract_blocks256(int npts, int ncomp) { return ncomp * ((npts + 255) / 256); }
+
+// Single-shot path: the shell's nshell workgroups, then the receiver extraction of the
+// same step in the workgroups past them (one launch fewer per step).
+__global__ __launch_bounds__(256) void acoustic_boundary_ext(const BShot* __restrict__ T, StepIdx it, int nshell) {
+    const int L = blockIdx.x;
+    const int k = it.value() - T[0].t0;
+    if (k < 0 || k > T[0].nt - 2) return;
+    if (L >= nshell) {
+        extract_block256(T[0], L - nshell, k);
+        return;
+    }
+    AcuArgs A = T[0].a;
+    bshot_rotate(A, T[0], k);
+    acoustic_boundary_block(A, L, T[0].nshell_pts);
+}
+
 // Occupancy: elastic 5 waves per SIMD (96 VGPRs, 12 B/lane spilled) measured faster than
 // the compiler's 4 waves without spills; VTI (more model fields per point) is faster at 4.
 template <int TX, bool VTI>
@@ -136,6 +162,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void e
     bshot_rotate(A, T[s], k);
     const int l = L - m.first(s), nb = T[s].nshell_blk;
     elastic_boundary_block<VTI>(A, l % nb, l / nb, T[s].nshell_pts);
+}
+
+// Single-shot path: the shell's 3 nshell_blk workgroups, then the receiver extraction of
+// the same step (as acoustic_boundary_ext).
+template <bool VTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void elastic_boundary_ext(
+    const BShot* __restrict__ T, StepIdx it) {
+    const int L = blockIdx.x;
+    const int k = it.value() - T[0].t0;
+    if (k < 0 || k > T[0].nt - 2) return;
+    const int nb = T[0].nshell_blk;
+    if (L >= 3 * nb) {
+        extract_block256(T[0], L - 3 * nb, k);
+        return;
+    }
+    EArgs A = T[0].ef;
+    bshot_rotate(A, T[0], k);
+    elastic_boundary_block<VTI>(A, L % nb, L / nb, T[0].nshell_pts);
 }
 
 // Slabs of the shell away from its edges, 256 threads per workgroup: a shot's first

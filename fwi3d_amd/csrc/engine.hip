@@ -524,16 +524,17 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     // fields of local step k: prev = F[k % 3], curr = F[(k + 1) % 3] (the kernels' rotation)
     auto field = [&](int k) { return F + (long long)(k % 3) * n; };
     // one time step: receivers read curr, interior and shell write next, sources add to it
+    // receivers are extracted by workgroups of the shell launch
+    const int ext_blk = (!adj && E.npts > 0) ? extract_blocks256(E.npts, 1) : 0;
     auto step = [&](StepIdx si) {
-        if (!adj && E.npts > 0)
-            hipLaunchKernelGGL(extract_batch, dim3(T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, si);
         timed(c, 4, [&] {
             if (nblk > 0) timed(c, 0, [&] {
                 if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, si);
                 else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, si);
             });
             timed(c, 2, [&] {
-                hipLaunchKernelGGL(acoustic_boundary_batch, dim3(T1h.nshell_blk), dim3(256), 0, c->stream, T1, BMap{}, si);
+                hipLaunchKernelGGL(acoustic_boundary_ext, dim3(T1h.nshell_blk + ext_blk), dim3(256), 0, c->stream, T1, si,
+                                   T1h.nshell_blk);
             });
         });
         if (I.ntgt > 0)
@@ -569,8 +570,10 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 // One elastic/VTI time step: flux pre-pass, interior y-march, boundary shell.
 // The one-shot table T carries the flux pass's tiling (whole grid, nflux workgroups)
 // and the interior's (nint workgroups, 0 when the interior is empty).
+// next_blk: receiver-extraction workgroups launched with the shell (0: none this step).
 template <int TX, bool VTI>
-void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint, int nshell_blk, int nzslab) {
+void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint, int nshell_blk, int nzslab,
+                  int next_blk) {
     timed(c, 1, [&] {
         hipLaunchKernelGGL((elastic_flux_batch<TX, VTI>), dim3(nflux), dim3(64, TX), 0, c->stream, T, BMap{}, it);
     });
@@ -579,7 +582,7 @@ void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint,
             hipLaunchKernelGGL((elastic_interior_batch<TX, VTI>), dim3(nint), dim3(64, TX), 0, c->stream, T, BMap{}, it);
         });
     timed(c, 2, [&] {
-        hipLaunchKernelGGL(elastic_boundary_batch<VTI>, dim3(3 * nshell_blk), dim3(256), 0, c->stream, T, BMap{}, it);
+        hipLaunchKernelGGL(elastic_boundary_ext<VTI>, dim3(3 * nshell_blk + next_blk), dim3(256), 0, c->stream, T, it);
     });
     if (nzslab > 0)
         timed(c, 3, [&] {
@@ -665,16 +668,16 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     // one time step: receivers read curr; flux, interior, shell and z slabs write next;
     // sources add to it (the batch kernels over the one-shot table: a batched shot runs
     // the same code)
+    // receivers are extracted by workgroups of the shell launch
+    const int ext_blk = (!adj && E.npts > 0) ? extract_blocks256(E.npts, 3) : 0;
     auto step = [&](StepIdx si) {
-        if (!adj && E.npts > 0)
-            hipLaunchKernelGGL(extract_batch, dim3(3 * T1h.next_blk), dim3(128), 0, c->stream, T1, BMap{}, si);
         timed(c, 4, [&] {
             if (vti) {
-                if (TX == 8) step_elastic<8, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
-                else step_elastic<4, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
+                if (TX == 8) step_elastic<8, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
+                else step_elastic<4, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
             } else {
-                if (TX == 8) step_elastic<8, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
-                else step_elastic<4, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab);
+                if (TX == 8) step_elastic<8, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
+                else step_elastic<4, false>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
             }
         });
         if (I.ntgt > 0) {
