@@ -140,7 +140,7 @@ __device__ __forceinline__ float acu_scale(const Geom& g, const int* bc, float K
 __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b, int npts) {
     const Geom& g = A.g;
     int ix, iy, iz;
-    const int t = b * 256 + threadIdx.x;
+    const int t = b * 256 + threadIdx.y * blockDim.x + threadIdx.x;  // 256 x 1 or 64 x 4 threads
     if (t >= npts || !shell_point(t, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
     const float* __restrict__ curr = A.curr;
     const float* __restrict__ bb = A.b;

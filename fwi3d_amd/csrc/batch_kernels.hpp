@@ -92,16 +92,40 @@ __global__ __launch_bounds__(256) void acoustic_boundary_batch(const BShot* __re
     acoustic_boundary_block(A, L - m.first(s), T[s].nshell_pts);
 }
 
+
 // Receivers of the single shot T[0] at local step k (they read curr, which no kernel of
 // the step writes), 256 per workgroup, component-major.
 __device__ __forceinline__ void extract_block256(const BShot& S, int l, int k) {
     const int nb = (S.E.npts + 255) / 256;
-    const int r = (l % nb) * 256 + threadIdx.x;
+    const int r = (l % nb) * 256 + threadIdx.y * blockDim.x + threadIdx.x;  // 256 x 1 or 64 x 4 threads
     if (r >= S.E.npts) return;
     const float* curr = S.F + (long long)((k + 1) % 3) * S.nf;
     extract_one(curr, S.g, S.E, S.rcv, S.nt, k, r, l / nb);
 }
 __host__ __device__ inline int extract_blocks256(int npts, int ncomp) { return ncomp * ((npts + 255) / 256); }
+
+// Single-shot acoustic step in one launch of 64 x 4 threads per workgroup (the interior's
+// TX = 4 tile): the nint interior tiles (XCD-remapped among themselves), then the nshell
+// shell workgroups, then the receiver extraction of the same step. All three write
+// disjoint outputs and read only curr, prev and the model.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void acoustic_step_ext(
+    const BShot* __restrict__ T, StepIdx it, int nint, int nshell) {
+    const int k = it.value() - T[0].t0;
+    if (k < 0 || k > T[0].nt - 2) return;
+    const int b = blockIdx.x;
+    if (b >= nint + nshell) {
+        extract_block256(T[0], b - nint - nshell, k);
+        return;
+    }
+    AcuArgs A = T[0].a;
+    bshot_rotate(A, T[0], k);
+    if (b < nint) {
+        const int L = xcd_remap(b, nint);
+        acoustic_interior_tile<4>(A, [&] { return L; });
+    } else {
+        acoustic_boundary_block(A, b - nint, T[0].nshell_pts);
+    }
+}
 
 // Single-shot path: the shell's nshell workgroups, then the receiver extraction of the
 // same step in the workgroups past them (one launch fewer per step).

@@ -526,8 +526,16 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     // one time step: receivers read curr, interior and shell write next, sources add to it
     // receivers are extracted by workgroups of the shell launch
     const int ext_blk = (!adj && E.npts > 0) ? extract_blocks256(E.npts, 1) : 0;
+    // with 4-row tiles the interior, shell and receivers share one launch, except when
+    // every kernel is timed on its own (fwi3d_profile level 1)
+    const bool one_launch = TX == 4 && nblk > 0 && !(c->prof && !c->prof_steps_only);
     auto step = [&](StepIdx si) {
         timed(c, 4, [&] {
+            if (one_launch) {
+                hipLaunchKernelGGL(acoustic_step_ext, dim3(nblk + T1h.nshell_blk + ext_blk), dim3(64, 4), 0, c->stream,
+                                   T1, si, nblk, T1h.nshell_blk);
+                return;
+            }
             if (nblk > 0) timed(c, 0, [&] {
                 if (TX == 8) hipLaunchKernelGGL(acoustic_interior_batch<8>, dim3(nblk), dim3(64, 8), 0, c->stream, T1, BMap{}, si);
                 else hipLaunchKernelGGL(acoustic_interior_batch<4>, dim3(nblk), dim3(64, 4), 0, c->stream, T1, BMap{}, si);
