@@ -125,3 +125,31 @@ def test_acoustic_gradient_matches_oracle(eng, sub):
         e = rel(grad[k], ograd[k])
         print(f"  grad[{k}] rel L2 {e:.2e}")
         assert e <= GRAD_TOL
+
+
+@pytest.mark.parametrize("shape,case", [((19, 17, 45), "mixed"), ((19, 26, 70), "free+absorb+taper")])
+def test_acoustic_every_point_matches_oracle(eng, shape, case):
+    """Receivers on every grid node (quadratic extraction weights reduce to the node
+    value): the whole pressure field at every step against the oracle, through the
+    single-launch step (interior tiles, shell and receivers in one launch) on odd shapes."""
+    nt = 60
+    model, src, sloc, _, d = small_shot(shape=shape, nt=nt)
+    ny, nx, nz = shape
+    h = d[0]
+    iy, ix, iz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    rloc = np.stack([ix.ravel() * h, iy.ravel() * h, iz.ravel() * h], axis=1).astype(np.float32)
+    cfg = CASES[case]
+    dt = 0.001
+    print(f"[acoustic every point] shape={shape} case={case} nt={nt}", flush=True)
+    inj, ext = F.Injector(0, sloc, shape, d), F.Injector(0, rloc, shape, d)
+    par = F.make_param(nt, dt, bc=cfg["bc"], taper=cfg["taper"])
+    got = eng.propagate("acoustic", model, src, inj, ext, par, d)
+    oinj, oext = O.Injector(0, sloc, shape, d, (0, 0, 0)), O.Injector(0, rloc, shape, d, (0, 0, 0))
+    opar = O.make_par(nt, 0, dt, bc=cfg["bc"], taper=cfg["taper"])
+    want, _ = O.propagate("acoustic", model, src, oinj, oext, opar, d, s="f")
+    e = rel(got, want)
+    last = np.abs(got[:, :, -1] - want[:, :, -1]).max() / np.abs(want[:, :, -1]).max()
+    print(f"  rel L2 vs oracle f32 {e:.2e}, worst point of the last sample {last:.2e}")
+    assert np.isfinite(got).all() and np.abs(want[:, :, -1]).max() > 0
+    assert e <= TRACE_TOL
+    assert last <= 10 * TRACE_TOL
