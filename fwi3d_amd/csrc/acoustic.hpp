@@ -279,10 +279,6 @@ __device__ __forceinline__ void extract_one(const float* __restrict__ u3, const 
                 acc += yw[a] * xw[bb] * zw[c] * u[g.at(I.xi[3 * r + bb], I.yi[3 * r + a], I.zi[3 * r + c])];
     rcv[(long long)(comp * I.npts + r) * nt + it] = acc;
 }
-__global__ void extract_points(const float* __restrict__ u3, Geom g, InjTab I, float* __restrict__ rcv, int nt, int it) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < I.npts) extract_one(u3, g, I, rcv, nt, it, r, blockIdx.y);
-}
 
 // Imaging condition of nl_we_op_a::compute_gradients (we_op.cpp:1555-1605), FP64
 // accumulation. Snapshot views of slots itl-1, itl, itl+1 (a slot past the store has a
