@@ -231,6 +231,37 @@ __global__ __launch_bounds__(8 * TXZ) void elastic_zslab_batch(const BShot* __re
     }
 }
 
+// Single-shot path: the slab kernel's nslab workgroups (z slabs, then side slabs), then
+// the shell's edge workgroups (3 components), then the receiver extraction of the same
+// step, in one launch: the slab, edge and receiver work write disjoint outputs, and the
+// edge workgroups fill the tail of the latency-bound slab workgroups.
+template <bool VTI>
+__global__ __launch_bounds__(256) void elastic_edges_slabs_ext(const BShot* __restrict__ T, StepIdx it, int nslab) {
+    __shared__ float smem[slab_lds_floats<VTI>()];
+    const int k = it.value() - T[0].t0;
+    if (k < 0 || k > T[0].nt - 2) return;
+    const int L = blockIdx.x;
+    EArgs A = T[0].ef;
+    bshot_rotate(A, T[0], k);
+    if (L < nslab) {
+        const ZSlabTiles Z = T[0].zs;
+        if (L < Z.nzs) {
+            elastic_zslab_tile<32, VTI>(A, Z, [&] { return L; }, smem);
+        } else {
+            const int r = L - Z.nzs, n0 = 2 * Z.sntz * Z.snc0;
+            if (r < n0) elastic_side_tile<0, VTI>(A, Z, r, smem);
+            else elastic_side_tile<1, VTI>(A, Z, r - n0, smem);
+        }
+        return;
+    }
+    const int e = L - nslab, nb = T[0].nshell_blk;
+    if (e >= 3 * nb) {
+        extract_block256(T[0], e - 3 * nb, k);
+        return;
+    }
+    elastic_boundary_block<VTI>(A, e % nb, e / nb, T[0].nshell_pts);
+}
+
 // ---- source injection into `next` after the step (mode 0), receivers before it.
 __global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;

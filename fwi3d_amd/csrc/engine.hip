@@ -589,6 +589,13 @@ void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint,
         timed(c, 0, [&] {
             hipLaunchKernelGGL((elastic_interior_batch<TX, VTI>), dim3(nint), dim3(64, TX), 0, c->stream, T, BMap{}, it);
         });
+    if (nzslab > 0 && !(c->prof && !c->prof_steps_only)) {
+        // slabs, edges and receivers in one launch (separate launches when every kernel is
+        // timed on its own, fwi3d_profile level 1)
+        hipLaunchKernelGGL(elastic_edges_slabs_ext<VTI>, dim3(nzslab + 3 * nshell_blk + next_blk), dim3(256), 0,
+                           c->stream, T, it, nzslab);
+        return;
+    }
     timed(c, 2, [&] {
         hipLaunchKernelGGL(elastic_boundary_ext<VTI>, dim3(3 * nshell_blk + next_blk), dim3(256), 0, c->stream, T, it);
     });
