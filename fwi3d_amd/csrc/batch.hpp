@@ -232,8 +232,10 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
     go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
     go(B::C_ESHELL, elastic_boundary_batch<false>, 0, dim3(256));
     go(B::C_VSHELL, elastic_boundary_batch<true>, 0, dim3(256));
-    go(B::C_EZS, elastic_zslab_batch<kZSlabTX, false>, 0, dim3(8 * kZSlabTX));
-    go(B::C_VZS, elastic_zslab_batch<kZSlabTX, true>, 0, dim3(8 * kZSlabTX));
+    // slab tiles and shell edges share one launch per physics (class C_EZS / C_VZS; the
+    // shell classes stay empty)
+    go(B::C_EZS, elastic_edges_slabs_batch<false>, 0, dim3(256));
+    go(B::C_VZS, elastic_edges_slabs_batch<true>, 0, dim3(256));
     go(B::C_AINJ, acoustic_inject_batch, 0, dim3(256));
     go(B::C_EINJ, elastic_inject_batch<false>, 0, dim3(256));
     go(B::C_VINJ, elastic_inject_batch<true>, 0, dim3(256));
@@ -442,8 +444,8 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = ni;
             d.nshell_pts = shell_edge_count(g);
             d.nshell_blk = (d.nshell_pts + 255) / 256;
-            s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 3 * d.nshell_blk;
-            s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g);
+            s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 0;  // the edges run in the slab class
+            s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g) + 3 * d.nshell_blk;
             s.nblk[grp == 1 ? B::C_EINJ : B::C_VINJ] = 3 * d.ninj_blk;
             A.prev = F; A.curr = F + 3 * n; A.next = F + 6 * n;
             if (d.I.ntgt > 0) {

@@ -262,6 +262,34 @@ __global__ __launch_bounds__(256) void elastic_edges_slabs_ext(const BShot* __re
     elastic_boundary_block<VTI>(A, e % nb, e / nb, T[0].nshell_pts);
 }
 
+// Batch form of the same: per shot, its slab workgroups, then its shell edge workgroups
+// (receivers are a class of their own in a batch).
+template <bool VTI>
+__global__ __launch_bounds__(256) void elastic_edges_slabs_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
+    __shared__ float smem[slab_lds_floats<VTI>()];
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it.value() - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const int l = L - m.first(s);
+    const ZSlabTiles Z = T[s].zs;
+    const int nslab = Z.nzs + 2 * Z.sntz * (Z.snc0 + Z.snc1);
+    if (l < nslab) {
+        if (l < Z.nzs) {
+            elastic_zslab_tile<32, VTI>(A, Z, [&] { return l; }, smem);
+        } else {
+            const int r = l - Z.nzs, n0 = 2 * Z.sntz * Z.snc0;
+            if (r < n0) elastic_side_tile<0, VTI>(A, Z, r, smem);
+            else elastic_side_tile<1, VTI>(A, Z, r - n0, smem);
+        }
+        return;
+    }
+    const int e = l - nslab, nb = T[s].nshell_blk;
+    elastic_boundary_block<VTI>(A, e % nb, e / nb, T[s].nshell_pts);
+}
+
 // ---- source injection into `next` after the step (mode 0), receivers before it.
 __global__ void acoustic_inject_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
     const int L = blockIdx.x;
