@@ -210,12 +210,19 @@ class Engine:
     def __init__(self, device=0):
         self._h = C.c_void_p()
         self._batches = []
+        self._servers = []
         rc = lib().fwi3d_create(C.byref(self._h), int(device))
         if rc != 0:
             raise RuntimeError(f"fwi3d_create failed ({rc})")
 
     def close(self):
         if self._h:
+            for s in list(self._servers):  # stop the server threads before the context goes
+                try:
+                    s.close()
+                except RuntimeError:  # a server that had failed; its requests already say so
+                    pass
+            self._servers = []
             for b in self._batches:  # the context destroys its batches with it
                 b._h = C.c_void_p()
             self._batches = []
@@ -567,6 +574,7 @@ class Server:
         self._shots = {}
         engine._check(lib().fwi3d_server_create(engine._h, int(hbm_budget), int(max_active), C.byref(self._h)),
                       "fwi3d_server_create")
+        engine._servers.append(self)
 
     def submit(self, kind, par, shot, model_ptr, src_ptr, data_ptr=None, obs_ptr=None, grad64_ptr=None):
         """kind REQ_MODEL: data <- apply_forward(model); REQ_GRADIENT: one FWI shot
@@ -610,6 +618,8 @@ class Server:
         if self._h:
             rc = lib().fwi3d_server_destroy(self._h)
             self._h = C.c_void_p()
+            if self in self.engine._servers:
+                self.engine._servers.remove(self)
             self.engine._check(rc, "fwi3d_server_destroy")
 
     def __del__(self):

@@ -59,7 +59,8 @@ struct fwi3d_batch {
     int next_id = 0;
     int TX = 4;       // x rows per workgroup of every y-march launch of the batch
     std::vector<Shot> shots;                 // admitted, not yet retired
-    std::vector<std::pair<int, hipEvent_t>> retired;  // id -> event after its last work
+    std::vector<std::pair<int, hipEvent_t>> retired;  // id -> event after its last work (pending)
+    std::vector<bool> done_ids;  // ids whose last work completed (their events are destroyed)
     bool dirty = true;
     fwi3d::DevBuf tab, maps;
     int cls_blocks[NCLS] = {};
@@ -484,6 +485,23 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
     }
 }
 
+// Drop the retired shots whose last work has completed: their events are destroyed and
+// only their ids are kept (a bit each), so a long-lived batch (a server's) holds events
+// for the shots still in flight only.
+void batch_prune_retired(fwi3d_batch* b) {
+    for (size_t k = 0; k < b->retired.size();) {
+        if (hipEventQuery(b->retired[k].second) != hipSuccess) {  // not ready, or an error the query reports
+            k++;
+            continue;
+        }
+        const int id = b->retired[k].first;
+        if ((size_t)id >= b->done_ids.size()) b->done_ids.resize((size_t)id + 1, false);
+        b->done_ids[(size_t)id] = true;
+        (void)hipEventDestroy(b->retired[k].second);
+        b->retired.erase(b->retired.begin() + (long)k);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -546,6 +564,7 @@ int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {
         REQUIRE(nsteps >= 0, "nsteps must be >= 0");
         HIP_OK(hipSetDevice(c->device));
         b->last_retired.clear();
+        batch_prune_retired(b);
         for (int q = 0; q < nsteps && !b->shots.empty(); q++) {
             const bool tuning = b->tune_idx >= 0;
             if (tuning && b->tune_left == b->tune_window) {  // a candidate's window starts
@@ -684,9 +703,10 @@ int fwi3d_batch_query(fwi3d_batch* b, int id, int* state) {
             if (e == hipSuccess) *state = 1;
             else if (e == hipErrorNotReady) *state = 0;
             else return fail(b->c, Err{FWI3D_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(e)});
+            if (e == hipSuccess) batch_prune_retired(b);
             return FWI3D_OK;
         }
-    *state = -1;
+    *state = (id >= 0 && (size_t)id < b->done_ids.size() && b->done_ids[(size_t)id]) ? 1 : -1;
     return FWI3D_OK;
 }
 

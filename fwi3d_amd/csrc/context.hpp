@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,18 +50,31 @@ struct DevBuf {
     T* as() const { return (T*)p; }
 };
 
+// Message of the last error raised by a C-ABI call on the calling thread. Code that
+// calls an entry point and turns its failure into an exception reads this, not the
+// context's message, which another thread may overwrite at any time.
+inline std::string& thread_error() {
+    static thread_local std::string m;
+    return m;
+}
+
 }  // namespace fwi3d
 
 struct fwi3d_batch;
+struct fwi3d_server;
 
 struct fwi3d_ctx {
     int device = 0;
-    std::vector<fwi3d_batch*> batches;  // live batches on this context (destroyed with it)
+    std::vector<fwi3d_batch*> batches;    // live batches on this context (destroyed with it)
+    std::vector<fwi3d_server*> servers;   // live servers (stopped and destroyed before the batches)
     // one-shot table of the single-shot time step (a BShot record, engine.hip): host
     // bytes kept until the next propagate call, and its device copy
     std::vector<char> shot1_host;
     fwi3d::DevBuf shot1_dev;
     hipStream_t stream = nullptr;
+    // last error message on the context; a server thread and its clients both set it,
+    // so it is only touched through fwi3d::set_error / fwi3d_last_error under err_mu
+    std::mutex err_mu;
     std::string err;
     fwi3d::DevBuf fields, model, aux, snaps, tapers, injtab, exttab, tmpio;
     std::vector<int> injhost[2];  // host copies of the injection lists (source, receiver tables)
@@ -103,3 +117,14 @@ struct fwi3d_ctx {
     std::vector<int> ev_count;  // time steps covered by each event pair
     int timed_count = 1;        // time steps the next timed launch covers
 };
+
+namespace fwi3d {
+
+inline void set_error(fwi3d_ctx* c, const std::string& msg) {
+    thread_error() = msg;
+    if (!c) return;
+    std::lock_guard<std::mutex> lk(c->err_mu);
+    c->err = msg;
+}
+
+}  // namespace fwi3d

@@ -742,7 +742,7 @@ void validate(int physics, const fwi3d_param* par, int nx, int ny, int nz, float
 }
 
 int fail(fwi3d_ctx* c, const Err& e) {
-    if (c) c->err = e.msg;
+    fwi3d::set_error(c, e.msg);
     return e.code;
 }
 
@@ -776,6 +776,8 @@ int fwi3d_create(fwi3d_ctx** out, int device) {
 
 int fwi3d_destroy(fwi3d_ctx* c) {
     if (!c) return FWI3D_OK;
+    // servers first: each stops its thread, which steps a batch of this context
+    while (!c->servers.empty()) fwi3d_server_destroy(c->servers.back());
     while (!c->batches.empty()) fwi3d_batch_destroy(c->batches.back());
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -789,7 +791,15 @@ int fwi3d_destroy(fwi3d_ctx* c) {
     return FWI3D_OK;
 }
 
-const char* fwi3d_last_error(const fwi3d_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* fwi3d_last_error(const fwi3d_ctx* c) {
+    // a copy per calling thread: the context's message may be replaced by another
+    // thread (a server's) while the caller still reads the returned string
+    static thread_local std::string copy;
+    if (!c) return "null context";
+    std::lock_guard<std::mutex> lk(const_cast<fwi3d_ctx*>(c)->err_mu);
+    copy = c->err;
+    return copy.c_str();
+}
 
 int fwi3d_injector_tables(int kind, const float* loc, int npts, int nx, int ny, int nz, float dx, float dy, float dz,
                           float ox, float oy, float oz, int* xind, int* yind, int* zind, float* xw, float* yw,
@@ -989,7 +999,7 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             for (int k = 0; k < fwi3d_ctx::kProfClasses; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
             const int rc = fwi3d_propagate_device(c, physics, 0, m.as<float>(), m.as<float>(), nullptr, &none, &none,
                                                   nullptr, &par, nx, ny, nz, 0.01f, 0.01f, 0.01f);
-            if (rc != FWI3D_OK) throw Err{rc, c->err};
+            if (rc != FWI3D_OK) throw Err{rc, fwi3d::thread_error()};
             return c->prof_ms[4] / steps;  // whole time steps (the kernels overlap)
         };
         run(4, 8);  // warm-up

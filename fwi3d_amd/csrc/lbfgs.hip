@@ -224,7 +224,7 @@ struct Wolfe {
 };
 
 int fail(fwi3d_ctx* c, const Err& e) {
-    if (c) c->err = e.msg;
+    fwi3d::set_error(c, e.msg);
     return e.code;
 }
 

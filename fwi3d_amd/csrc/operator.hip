@@ -502,7 +502,7 @@ void propagate_or_throw(fwi3d_ctx* c, int physics, int adj, const float* model, 
                         const fwi3d_shot* s) {
     const int rc = fwi3d_propagate_device(c, physics, adj, model, src, rcv, inj, ext, g64, p, s->nx, s->ny, s->nz,
                                           s->dx, s->dy, s->dz);
-    if (rc != FWI3D_OK) throw Err{rc, c->err};
+    if (rc != FWI3D_OK) throw Err{rc, fwi3d::thread_error()};
 }
 
 // apply_forward (we_op.cpp:872-946): raw model and source wavelets at the data rate
@@ -578,7 +578,7 @@ void jacobianT_device(fwi3d_ctx* c, fwi3d_op_param* par, const fwi3d_shot* s, in
 }
 
 int fail(fwi3d_ctx* c, const Err& e) {
-    if (c) c->err = e.msg;
+    fwi3d::set_error(c, e.msg);
     return e.code;
 }
 

@@ -201,7 +201,7 @@ void srv_start(Srv* s, SReq* q) {
     int id = -1;
     const int rc = fwi3d_batch_add(s->b, q->d.physics, q->conv, q->srcp, q->rcvp, &q->si.t, &q->ri.t, &q->pp, sh->nx,
                                    sh->ny, sh->nz, sh->dx, sh->dy, sh->dz, &id);
-    if (rc != FWI3D_OK) throw Err{rc, s->c->err};
+    if (rc != FWI3D_OK) throw Err{rc, fwi3d::thread_error()};
     q->shot = id;
     s->by_shot[id] = q;
     q->phase = 2;
@@ -253,7 +253,7 @@ void srv_after_forward(Srv* s, SReq* q) {
     int id = -1;
     const int rc = fwi3d_batch_add_adjoint(s->b, q->shot, q->d.physics, q->conv, asrc, &q->ri.t, q->g64, &q->pp,
                                            sh->nx, sh->ny, sh->nz, sh->dx, sh->dy, sh->dz, &id);
-    if (rc != FWI3D_OK) throw Err{rc, s->c->err};
+    if (rc != FWI3D_OK) throw Err{rc, fwi3d::thread_error()};
     s->by_shot.erase(q->shot);
     q->shot = id;
     s->by_shot[id] = q;
@@ -381,7 +381,7 @@ void srv_loop(Srv* s) {
             if (nshots > 0) {
                 // one step per call, so that arriving requests join at the next step
                 int nact = 0;
-                if (fwi3d_batch_step(s->b, 1, &nact) != FWI3D_OK) throw Err{FWI3D_EDEVICE, s->c->err};
+                if (fwi3d_batch_step(s->b, 1, &nact) != FWI3D_OK) throw Err{FWI3D_EDEVICE, fwi3d::thread_error()};
                 int bstep = 0;
                 (void)fwi3d_batch_status(s->b, nullptr, nullptr, &bstep);
                 {
@@ -434,7 +434,7 @@ void srv_loop(Srv* s) {
 }
 
 int srv_fail(Srv* s, const Err& e) {
-    if (s && s->c) s->c->err = e.msg;
+    if (s) fwi3d::set_error(s->c, e.msg);
     return e.code;
 }
 
@@ -454,17 +454,18 @@ int fwi3d_server_create(fwi3d_ctx* c, long long hbm_budget, int max_active, fwi3
         s->budget = hbm_budget;
         s->max_active = max_active;
         const int rc = fwi3d_batch_create(c, &s->b);
-        if (rc != FWI3D_OK) throw Err{rc, c->err};
+        if (rc != FWI3D_OK) throw Err{rc, fwi3d::thread_error()};
         HIP_OK(hipStreamCreateWithFlags(&s->prep, hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&s->prep_ev, hipEventDisableTiming));
         HIP_OK(hipHostMalloc((void**)&s->keys_host, 64, hipHostMallocDefault));
         s->th = std::thread(srv_loop, s.get());
+        c->servers.push_back(s.get());
     } catch (const Err& e) {
         if (s->b) fwi3d_batch_destroy(s->b);
         if (s->prep) (void)hipStreamDestroy(s->prep);
         if (s->prep_ev) (void)hipEventDestroy(s->prep_ev);
         if (s->keys_host) (void)hipHostFree(s->keys_host);
-        c->err = e.msg;
+        fwi3d::set_error(c, e.msg);
         return e.code;
     }
     *out = s.release();
@@ -575,6 +576,8 @@ int fwi3d_server_destroy(fwi3d_server* s) {
         s->cv_work.notify_all();
     }
     if (s->th.joinable()) s->th.join();
+    auto& v = s->c->servers;
+    v.erase(std::remove(v.begin(), v.end(), s), v.end());
     (void)hipSetDevice(s->c->device);
     (void)hipStreamSynchronize(s->c->stream);
     (void)hipStreamSynchronize(s->prep);

@@ -18,6 +18,7 @@ typedef struct {
     double dt;
     int bc[6], taper[6];
     double taper_strength, free_surface_stiffness;
+    int lag;  // 1: lagged adjoint derivatives in the elastic/VTI imaging condition (the reference); 0: lag-free
 } orc_par;
 
 // Shot-operator parameters (src/include/param.hpp:8-71): in/out like the reference's _par.
@@ -31,6 +32,7 @@ typedef struct {
     double taper_strength, free_surface_stiffness;
     int mt, seismotype, soft_clip;
     double vpmin, vpmax, vsmin, vsmax, rhomin, rhomax, deltamin, deltamax, epsilonmin, epsilonmax, gammamin, gammamax;
+    int lag;  // as in orc_par
 } orc_op_par;
 
 // Geometry of one shot; positions are arrays of the variant's type.
@@ -74,6 +76,7 @@ orc::PropParams<T> to_par(const orc_par* p) {
     for (int i = 0; i < 6; i++) { q.bc[i] = p->bc[i]; q.taper[i] = p->taper[i]; }
     q.taper_strength = (T)p->taper_strength;
     q.free_surface_stiffness = (T)p->free_surface_stiffness;
+    q.lag = p->lag;
     return q;
 }
 
@@ -106,6 +109,7 @@ orc::OpParams<T> to_op(const orc_op_par* p) {
     q.vpmin = (T)p->vpmin; q.vpmax = (T)p->vpmax; q.vsmin = (T)p->vsmin; q.vsmax = (T)p->vsmax;
     q.rhomin = (T)p->rhomin; q.rhomax = (T)p->rhomax; q.deltamin = (T)p->deltamin; q.deltamax = (T)p->deltamax;
     q.epsilonmin = (T)p->epsilonmin; q.epsilonmax = (T)p->epsilonmax; q.gammamin = (T)p->gammamin; q.gammamax = (T)p->gammamax;
+    q.lag = p->lag;
     return q;
 }
 
@@ -258,6 +262,40 @@ orc::ShotGeom<T> to_shot(const orc_shot* s) {
     }                                                                                                                  \
     int orc_extract_##S(const orc_inj* inj, const T* u, int nt, int comp, int it, T* rcv, int nx, int ny, int nz) {   \
         orc::extract<T>(to_inj<T>(inj), u, nt, comp, it, rcv, orc::Grid{nx, ny, nz});                                  \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_spatial_##S(int physics, const T* curr, const T* prev, T* next, const T* model, const int* bc, int nx,    \
+                        int ny, int nz, T dx, T dy, T dz, T dt, T free_surface_stiffness) {                            \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        const size_t n = g.n();                                                                                        \
+        if (physics == orc::PHYS_ACOUSTIC) {                                                                           \
+            const T alpha = (T)0.2508560249 / free_surface_stiffness;                                                  \
+            orc::acoustic_volume<T>(curr, next, model + n, g, dx, dy, dz);                                             \
+            orc::acoustic_sats<T>(bc, curr, prev, next, model, model + n, g, dx, dy, dz, dt, alpha);                  \
+            return 0;                                                                                                  \
+        }                                                                                                              \
+        const bool vti = physics == orc::PHYS_VTI;                                                                     \
+        const orc::EMod<T> M = orc::emod<T>(vti, model, n);                                                            \
+        std::vector<T> c11b(vti ? n : 0), du(9 * n), tmp(n);                                                           \
+        for (size_t i = 0; i < c11b.size(); i++) c11b[i] = M.c11b(i);                                                  \
+        orc::elastic_volume<T>(vti, M, c11b.data(), curr, next, du.data(), tmp.data(), g, dx, dy, dz);                 \
+        orc::elastic_sats<T>(M, bc, curr, prev, next, g, dx, dy, dz, dt);                                              \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_grad_event_##S(int physics, const T* model, const T* snaps, int nslots, int itl, int nt, int sub, T dt,   \
+                           const T* adjf, const T* derivf, T* grad, int nx, int ny, int nz, T dx, T dy, T dz) {        \
+        orc::Grid g{nx, ny, nz};                                                                                       \
+        const size_t n = g.n();                                                                                        \
+        if (physics == orc::PHYS_ACOUSTIC) {                                                                           \
+            std::vector<T> tmp(6 * n);                                                                                 \
+            orc::acoustic_imaging<T>(model, snaps, nslots, itl, nt, sub, dt, adjf, tmp.data(), grad, g, dx, dy, dz);  \
+            return 0;                                                                                                  \
+        }                                                                                                              \
+        const bool vti = physics == orc::PHYS_VTI;                                                                     \
+        std::vector<T> du(9 * n), tmp(9 * n);                                                                          \
+        orc::all_derivatives<T>(derivf, du.data(), g, dx, dy, dz);                                                     \
+        orc::elastic_imaging<T>(vti, orc::emod<T>(vti, model, n), snaps, nslots, itl, nt, sub, dt, adjf, du.data(),   \
+                                tmp.data(), grad, g, dx, dy, dz);                                                      \
         return 0;                                                                                                      \
     }                                                                                                                  \
     int orc_propagate_##S(int physics, int adj, const T* model, const T* src, T* rcv, const orc_inj* inj,             \

@@ -29,7 +29,7 @@ class Inj(C.Structure):
 class Par(C.Structure):
     _fields_ = [("nt", C.c_int), ("sub", C.c_int), ("dt", C.c_double),
                 ("bc", C.c_int * 6), ("taper", C.c_int * 6),
-                ("taper_strength", C.c_double), ("free_surface_stiffness", C.c_double)]
+                ("taper_strength", C.c_double), ("free_surface_stiffness", C.c_double), ("lag", C.c_int)]
 
 
 def _ptr(a):
@@ -174,8 +174,11 @@ class Injector:
                    self.xw.ctypes.data, self.yw.ctypes.data, self.zw.ctypes.data)
 
 
-def make_par(nt, sub, dt, bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6, taper_strength=0.05, free_surface_stiffness=1.05):
-    return Par(nt, sub, dt, (C.c_int * 6)(*bc), (C.c_int * 6)(*taper), taper_strength, free_surface_stiffness)
+def make_par(nt, sub, dt, bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6, taper_strength=0.05, free_surface_stiffness=1.05,
+             lag=1):
+    """lag (test-only): 1 feeds the elastic/VTI imaging condition the previous step's
+    adjoint derivatives, as the reference does; 0 the current step's."""
+    return Par(nt, sub, dt, (C.c_int * 6)(*bc), (C.c_int * 6)(*taper), taper_strength, free_surface_stiffness, lag)
 
 
 def n_snap_slots(nt, sub):
@@ -206,6 +209,38 @@ def propagate(physics, model, src, inj, ext, par, d, adj=False, grad=None, snaps
     return rcv, snaps
 
 
+def spatial(physics, curr, prev, model, bc, d, dt, free_surface_stiffness=1.05, s="d"):
+    """The spatial operator of one time step as propagate applies it (volume terms and
+    the SATs of the faces, before the source, the leapfrog update and the absorbing
+    scaling): returns next."""
+    T = _dt(s)
+    cT = C.c_float if s == "f" else C.c_double
+    ph = PHYS[physics] if isinstance(physics, str) else physics
+    ny, nx, nz = model.shape[-3:]
+    curr, prev, model = (np.ascontiguousarray(a, dtype=T) for a in (curr, prev, model))
+    nxt = np.zeros_like(curr)
+    getattr(lib(), f"orc_spatial_{s}")(ph, _ptr(curr), _ptr(prev), _ptr(nxt), _ptr(model), (C.c_int * 6)(*bc),
+                                       nx, ny, nz, cT(d[0]), cT(d[1]), cT(d[2]), cT(dt), cT(free_surface_stiffness))
+    return nxt
+
+
+def grad_event(physics, model, snaps, itl, nt, sub, dt, adjf, derivf, grad, d, s="d"):
+    """One imaging event of propagate's adjoint (grad accumulates in place): forward
+    snapshots itl-1..itl+1 of the store `snaps`, the adjoint field adjf for the rho / K
+    term and the derivatives of derivf for the lambda / mu / 1/rho terms (elastic/VTI;
+    the acoustic event differentiates adjf)."""
+    T = _dt(s)
+    cT = C.c_float if s == "f" else C.c_double
+    ph = PHYS[physics] if isinstance(physics, str) else physics
+    ny, nx, nz = model.shape[-3:]
+    for a in (model, snaps, adjf, derivf, grad):
+        assert a.dtype == T and a.flags.c_contiguous
+    getattr(lib(), f"orc_grad_event_{s}")(ph, _ptr(model), _ptr(snaps), snaps.shape[0], int(itl), int(nt), int(sub),
+                                          cT(dt), _ptr(adjf), _ptr(derivf), _ptr(grad), nx, ny, nz,
+                                          cT(d[0]), cT(d[1]), cT(d[2]))
+    return grad
+
+
 # ---------------------------------------------------------------------------------
 # Shot operator (apply_forward / apply_jacobianT / FWI shot) of the restatement.
 
@@ -217,7 +252,8 @@ class OpPar(C.Structure):
                 ("taper_strength", C.c_double), ("free_surface_stiffness", C.c_double), ("mt", C.c_int),
                 ("seismotype", C.c_int), ("soft_clip", C.c_int)] + \
                [(k, C.c_double) for k in ("vpmin", "vpmax", "vsmin", "vsmax", "rhomin", "rhomax", "deltamin",
-                                          "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")]
+                                          "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")] + \
+               [("lag", C.c_int)]
 
 
 class ShotC(C.Structure):
@@ -232,7 +268,7 @@ OP_DEFAULTS = dict(nmodels=3, dt=-1.0, courant=0.5, sub=0, nt=0, vmin=0.1, vmax=
                    sinc_half_length=11, bc=(1,) * 6, taper=(0,) * 6, taper_strength=0.05,
                    free_surface_stiffness=1.05, mt=0, seismotype=0, soft_clip=0, vpmin=0.2, vpmax=8.0, vsmin=0.1,
                    vsmax=5.0, rhomin=0.1, rhomax=8.0, deltamin=-0.5, deltamax=1.0, epsilonmin=0.0, epsilonmax=1.0,
-                   gammamin=0.0, gammamax=1.0)
+                   gammamin=0.0, gammamax=1.0, lag=1)
 
 
 def op_par(**kw):

@@ -28,6 +28,7 @@ struct OpParams {
     int mt = 0, seismotype = 0, soft_clip = 0;
     T vpmin = (T)0.2, vpmax = 8, vsmin = (T)0.1, vsmax = 5, rhomin = (T)0.1, rhomax = 8;
     T deltamin = (T)-0.5, deltamax = 1, epsilonmin = 0, epsilonmax = 1, gammamin = 0, gammamax = 1;
+    int lag = 1;  // test-only: lagged (1, the reference) or lag-free (0) adjoint derivatives
     // outputs of analyze_model
     int nt = 0;
     T vmax = 0, vmin = 0;
@@ -236,6 +237,7 @@ PropParams<T> prop_params(const OpParams<T>& p) {
     for (int i = 0; i < 6; i++) { q.bc[i] = p.bc[i]; q.taper[i] = p.taper[i]; }
     q.taper_strength = p.taper_strength;
     q.free_surface_stiffness = p.free_surface_stiffness;
+    q.lag = p.lag;
     return q;
 }
 

@@ -41,6 +41,11 @@ struct PropParams {
     int taper[6] = {0, 0, 0, 0, 0, 0};  // same order
     T taper_strength = (T)0.05;
     T free_surface_stiffness = (T)1.05;
+    // Test-only switch: 1 (the reference's behaviour) feeds the elastic/VTI imaging
+    // condition the adjoint derivatives of the previous step (we_op.cpp:637-688); 0 takes
+    // them from the current adjoint field, the lag-free formula the finite-difference
+    // checks of the gradient need.
+    int lag = 1;
 };
 
 // Iterate the points of one face plane: f(p, ix, iy, iz).
@@ -351,6 +356,62 @@ void extract(const Injector<T>& I, const T* u, int nt, int comp, int it, T* rcv,
 inline int n_snap_slots(int nt, int sub) { return sub > 0 ? 1 + nt / sub : 0; }
 
 // ----------------------------------------------------------------------------------
+// Spatial operator of one acoustic step (we_op.cpp:1664-1766): the volume terms
+// sum_a D2_a(1/rho) curr overwrite `next`; the SATs of the faces add to it (they read
+// curr and, for absorbing faces, prev).
+template <class T>
+void acoustic_volume(const T* curr, T* next, const T* b, const Grid& g, T dx, T dy, T dz) {
+    d2<T>(AX_X, false, curr, next, b, g, dx, 1);
+    d2<T>(AX_Y, true, curr, next, b, g, dy, 1);
+    d2<T>(AX_Z, true, curr, next, b, g, dz, 1);
+}
+
+template <class T>
+void acoustic_sats(const int* bc, const T* curr, const T* prev, T* next, const T* K, const T* b, const Grid& g, T dx,
+                   T dy, T dz, T dt, T alpha) {
+    for (int f = 0; f < 6; f++) asat_face<T>(f, bc[f], curr, prev, next, K, b, g, dx, dy, dz, dt, alpha);
+}
+
+// Acoustic imaging condition of one event (we_op.cpp:1555-1605): snapshots itl-1, itl,
+// itl+1 of the forward field and the adjoint field; tmp holds 6 fields.
+template <class T>
+void acoustic_imaging(const T* K, const T* snaps, int nslots, int itl, int nt, int sub, T dt, const T* adjf, T* tmp,
+                      T* grad, const Grid& g, T dx, T dy, T dz) {
+    const size_t N = g.n();
+    const T sdt = (T)sub * dt;
+    const T* fn = snaps + (size_t)itl * N;
+    const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * N : nullptr;
+    const T* fp = itl + 1 < nslots ? snaps + (size_t)(itl + 1) * N : nullptr;
+    T* t = tmp;
+    d1<T>(AX_X, false, fn, t + 0 * N, g, dx);
+    d1<T>(AX_Y, false, fn, t + 1 * N, g, dy);
+    d1<T>(AX_Z, false, fn, t + 2 * N, g, dz);
+    d1<T>(AX_X, false, adjf, t + 3 * N, g, dx);
+    d1<T>(AX_Y, false, adjf, t + 4 * N, g, dy);
+    d1<T>(AX_Z, false, adjf, t + 5 * N, g, dz);
+    T* gK = grad;
+    T* gB = grad + N;
+    const int last = nt / sub;
+#pragma omp parallel for
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        size_t i = (size_t)ii;
+        T dot = t[3 * N + i] * t[i] + t[4 * N + i] * t[N + i] + t[5 * N + i] * t[2 * N + i];
+        T k2 = sdt * K[i] * K[i];
+        const T fpv = fp ? fp[i] : (T)0;
+        if (itl > 0 && itl < last) {
+            gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - 2 * fn[i] + fm[i]);
+            gB[i] += sdt * dot;
+        } else if (itl == 0) {
+            gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - fn[i]);
+            gB[i] += (T)0.5 * sdt * dot;
+        } else {
+            gK[i] -= (T)1.0 / k2 * adjf[i] * (-2 * fn[i] + fm[i]);
+            gB[i] += sdt * dot;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------
 // Acoustic propagation. model = {K, 1/rho}; src [1][ns][nt]; rcv [1][nr][nt];
 // grad [2][N] (gK, g(1/rho)) accumulated when non-null (adjoint); snaps [slots][N].
 template <class T>
@@ -374,38 +435,8 @@ void propagate_acoustic(bool adj, const T* model, const T* src, T* rcv, const In
     for (size_t i = 0; i < N; i++) prev[i] *= (T)0.5 * K[i] * dt * dt;
 
     auto grad_event = [&](int itl, const T* adjf) {
-        // imaging condition with snapshots itl-1, itl, itl+1 and the adjoint field
-        const T sdt = (T)par.sub * dt;
-        const T* fn = snaps + (size_t)itl * N;
-        const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * N : nullptr;
-        const T* fp = itl + 1 < n_snap_slots(nt, par.sub) ? snaps + (size_t)(itl + 1) * N : nullptr;
-        T* t = tmp.data();
-        d1<T>(AX_X, false, fn, t + 0 * N, g, dx);
-        d1<T>(AX_Y, false, fn, t + 1 * N, g, dy);
-        d1<T>(AX_Z, false, fn, t + 2 * N, g, dz);
-        d1<T>(AX_X, false, adjf, t + 3 * N, g, dx);
-        d1<T>(AX_Y, false, adjf, t + 4 * N, g, dy);
-        d1<T>(AX_Z, false, adjf, t + 5 * N, g, dz);
-        T* gK = grad;
-        T* gB = grad + N;
-        const int last = nt / par.sub;
-#pragma omp parallel for
-        for (long long ii = 0; ii < (long long)N; ii++) {
-            size_t i = (size_t)ii;
-            T dot = t[3 * N + i] * t[i] + t[4 * N + i] * t[N + i] + t[5 * N + i] * t[2 * N + i];
-            T k2 = sdt * K[i] * K[i];
-            const T fpv = fp ? fp[i] : (T)0;
-            if (itl > 0 && itl < last) {
-                gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - 2 * fn[i] + fm[i]);
-                gB[i] += sdt * dot;
-            } else if (itl == 0) {
-                gK[i] -= (T)1.0 / k2 * adjf[i] * (fpv - fn[i]);
-                gB[i] += (T)0.5 * sdt * dot;
-            } else {
-                gK[i] -= (T)1.0 / k2 * adjf[i] * (-2 * fn[i] + fm[i]);
-                gB[i] += sdt * dot;
-            }
-        }
+        acoustic_imaging<T>(K, snaps, n_snap_slots(nt, par.sub), itl, nt, par.sub, dt, adjf, tmp.data(), grad, g, dx,
+                            dy, dz);
     };
 
     for (int it = 0; it < nt - 1; it++) {
@@ -413,11 +444,9 @@ void propagate_acoustic(bool adj, const T* model, const T* src, T* rcv, const In
         if (!grad) extract(ext, curr, nt, 0, it, rcv, g);
         if (grad && par.sub > 0 && (nt - 1 - it) % par.sub == 0 && it != 0) grad_event((nt - 1 - it) / par.sub, curr);
 
-        d2<T>(AX_X, false, curr, next, b, g, dx, 1);
-        d2<T>(AX_Y, true, curr, next, b, g, dy, 1);
-        d2<T>(AX_Z, true, curr, next, b, g, dz, 1);
+        acoustic_volume<T>(curr, next, b, g, dx, dy, dz);
         inject(inj, src, nt, 0, it, next, g);
-        for (int f = 0; f < 6; f++) asat_face<T>(f, par.bc[f], curr, prev, next, K, b, g, dx, dy, dz, dt, alpha);
+        acoustic_sats<T>(par.bc, curr, prev, next, K, b, g, dx, dy, dz, dt, alpha);
 #pragma omp parallel for
         for (long long ii = 0; ii < (long long)N; ii++) {
             size_t i = (size_t)ii;
@@ -434,6 +463,170 @@ void propagate_acoustic(bool adj, const T* model, const T* src, T* rcv, const In
 }
 
 // ----------------------------------------------------------------------------------
+// Spatial operator of one elastic / VTI step (we_op.cpp:646-801, 1290-1465). The volume
+// terms overwrite `next`: second-derivative terms D_aa(b_{c,a}) u_c, normal-stress terms
+// D_c(sigma_cc'), mixed terms D_a(G D_c u_a). du[a*3 + c] receives D_a u_c of curr (the
+// derivatives a lagged imaging event reads in the next step); tmp holds one field.
+template <class T>
+void elastic_volume(bool vti, const EMod<T>& M, const T* c11b, const T* curr, T* next, T* du, T* tmp, const Grid& g,
+                    T dx, T dy, T dz) {
+    const size_t N = g.n();
+    auto DU = [&](int a, int c) { return du + (size_t)(a * 3 + c) * N; };
+    const T* lam = M.m[0];
+    const T* mu = M.m[1];
+    // second-derivative terms: D_aa(b_{c,a}) u_c
+    for (int c = 0; c < 3; c++) {
+        for (int a = 0; a < 3; a++) {
+            const T h = (T)h_of(a, dx, dy, dz);
+            T* out = next + c * N;
+            const T* in = curr + c * N;
+            const bool add = (a != AX_X);
+            if (!vti) {
+                d2<T>(a, add, in, out, mu, g, h, a == c ? (T)2 : (T)1);
+            } else {
+                // VTI: x-x and y-y use c11b for the matching component, c66 across x/y
+                if ((c == 0 && a == AX_X) || (c == 1 && a == AX_Y)) d2<T>(a, add, in, out, c11b, g, h, (T)1);
+                else if ((c == 0 && a == AX_Y) || (c == 1 && a == AX_X)) d2<T>(a, add, in, out, M.m[5], g, h, 1);
+                else d2<T>(a, add, in, out, mu, g, h, (a == AX_Z && c == 2) ? (T)2 : (T)1);
+            }
+        }
+    }
+    // normal-stress terms: next_c += D_c( sigma_cc' )
+    d1<T>(AX_X, false, curr + 0 * N, DU(AX_X, 0), g, dx);
+    d1<T>(AX_Y, false, curr + 1 * N, DU(AX_Y, 1), g, dy);
+    d1<T>(AX_Z, false, curr + 2 * N, DU(AX_Z, 2), g, dz);
+    {
+        const T *ux = DU(AX_X, 0), *uy = DU(AX_Y, 1), *uz = DU(AX_Z, 2);
+        T* t = tmp;
+        if (!vti) {
+            for (size_t i = 0; i < N; i++) t[i] = lam[i] * (ux[i] + uy[i] + uz[i]);
+            d1<T>(AX_X, true, t, next + 0 * N, g, dx);
+            d1<T>(AX_Y, true, t, next + 1 * N, g, dy);
+            d1<T>(AX_Z, true, t, next + 2 * N, g, dz);
+        } else {
+            for (size_t i = 0; i < N; i++) t[i] = M.c11c(i) * ux[i] + M.c12(i) * uy[i] + M.c13(i) * uz[i];
+            d1<T>(AX_X, true, t, next + 0 * N, g, dx);
+            for (size_t i = 0; i < N; i++) t[i] = M.c11c(i) * uy[i] + M.c12(i) * ux[i] + M.c13(i) * uz[i];
+            d1<T>(AX_Y, true, t, next + 1 * N, g, dy);
+            for (size_t i = 0; i < N; i++) t[i] = M.c13(i) * (ux[i] + uy[i]) + lam[i] * uz[i];
+            d1<T>(AX_Z, true, t, next + 2 * N, g, dz);
+        }
+    }
+    // mixed terms: next_c += D_a( G * D_c u_a ), a != c.  The z-derivative of the
+    // x-component term uses dy as its spacing (we_op.cpp:678, 1339).
+    const T* gxy = vti ? M.m[5] : mu;
+    d1<T>(AX_X, false, curr + 1 * N, DU(AX_X, 1), g, dx);
+    d1<T>(AX_X, false, curr + 2 * N, DU(AX_X, 2), g, dx);
+    d1<T>(AX_Y, true, DU(AX_X, 1), next + 0 * N, g, dy, gxy);
+    d1<T>(AX_Z, true, DU(AX_X, 2), next + 0 * N, g, dy, mu);
+    d1<T>(AX_Y, false, curr + 0 * N, DU(AX_Y, 0), g, dy);
+    d1<T>(AX_Y, false, curr + 2 * N, DU(AX_Y, 2), g, dy);
+    d1<T>(AX_X, true, DU(AX_Y, 0), next + 1 * N, g, dx, gxy);
+    d1<T>(AX_Z, true, DU(AX_Y, 2), next + 1 * N, g, dz, mu);
+    d1<T>(AX_Z, false, curr + 0 * N, DU(AX_Z, 0), g, dz);
+    d1<T>(AX_Z, false, curr + 1 * N, DU(AX_Z, 1), g, dz);
+    d1<T>(AX_X, true, DU(AX_Z, 0), next + 2 * N, g, dx, mu);
+    d1<T>(AX_Y, true, DU(AX_Z, 1), next + 2 * N, g, dy, mu);
+}
+
+// Traction SATs of the faces with bc 1 (free surface) or 2 (absorbing), added to next.
+template <class T>
+void elastic_sats(const EMod<T>& M, const int* bc, const T* curr, const T* prev, T* next, const Grid& g, T dx, T dy,
+                  T dz, T dt) {
+    const size_t N = g.n();
+    const T* cu[3] = {curr, curr + N, curr + 2 * N};
+    const T* pv[3] = {prev, prev + N, prev + 2 * N};
+    T* nx3[3] = {next, next + N, next + 2 * N};
+    for (int f = 0; f < 6; f++)
+        if (bc[f] == 1 || bc[f] == 2) esat_face<T>(f, bc[f], cu, pv, nx3, M, g, dx, dy, dz, dt);
+}
+
+// Elastic / VTI imaging condition of one event (we_op.cpp:488-565, 1108-1216): snapshots
+// itl-1, itl, itl+1 of the forward field, the adjoint field adjf (rho term) and the
+// adjoint derivatives adu[a*3 + c] = D_a u_c (lambda and mu terms; the reference passes
+// those of the previous step). tmp holds 9 fields.
+template <class T>
+void elastic_imaging(bool vti, const EMod<T>& M, const T* snaps, int nslots, int itl, int nt, int sub, T dt,
+                     const T* adjf, const T* adu, T* tmp, T* grad, const Grid& g, T dx, T dy, T dz) {
+    const size_t N = g.n();
+    const T sdt = (T)sub * dt;
+    const T* fn = snaps + (size_t)itl * 3 * N;
+    const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * 3 * N : nullptr;
+    const T* fp = itl + 1 < nslots ? snaps + (size_t)(itl + 1) * 3 * N : nullptr;
+    T* t = tmp;
+    // forward-field derivatives: ux_x, uy_y, uz_z, uz_x, ux_z, uy_x, ux_y, uz_y, uy_z
+    d1<T>(AX_X, false, fn + 0 * N, t + 0 * N, g, dx);
+    d1<T>(AX_Y, false, fn + 1 * N, t + 1 * N, g, dy);
+    d1<T>(AX_Z, false, fn + 2 * N, t + 2 * N, g, dz);
+    d1<T>(AX_X, false, fn + 2 * N, t + 3 * N, g, dx);
+    d1<T>(AX_Z, false, fn + 0 * N, t + 4 * N, g, dz);
+    d1<T>(AX_X, false, fn + 1 * N, t + 5 * N, g, dx);
+    d1<T>(AX_Y, false, fn + 0 * N, t + 6 * N, g, dy);
+    d1<T>(AX_Y, false, fn + 2 * N, t + 7 * N, g, dy);
+    d1<T>(AX_Z, false, fn + 1 * N, t + 8 * N, g, dz);
+    T* g0 = grad;
+    T* g1 = grad + N;
+    T* g2 = grad + 2 * N;
+    const int last = nt / sub;
+    const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
+    const T w = branch == 0 ? (T)0.5 : (T)1;
+    auto DU = [&](int a, int c) { return adu + (size_t)(a * 3 + c) * N; };
+    const T *ax_x = DU(AX_X, 0), *ay_y = DU(AX_Y, 1), *az_z = DU(AX_Z, 2);
+    const T *az_x = DU(AX_Z, 0), *ax_z = DU(AX_X, 2), *ay_x = DU(AX_Y, 0), *ax_y = DU(AX_X, 1);
+    const T *az_y = DU(AX_Z, 1), *ay_z = DU(AX_Y, 2);
+#pragma omp parallel for
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        size_t i = (size_t)ii;
+        T rt = 0;
+        for (int c = 0; c < 3; c++) {
+            size_t q = c * N + i;
+            const T fpv = fp ? fp[q] : (T)0;
+            T d2t = branch == 1 ? fpv - 2 * fn[q] + fm[q] : (branch == 0 ? fpv - fn[q] : -fn[q] + fm[q]);
+            rt += adjf[q] * d2t;
+        }
+        if (!vti) {
+            g0[i] += w * sdt * (ax_x[i] + ay_y[i] + az_z[i]) * (t[i] + t[N + i] + t[2 * N + i]);
+            g1[i] += w * sdt * ((az_x[i] + ax_z[i]) * (t[3 * N + i] + t[4 * N + i]) +
+                                (ay_x[i] + ax_y[i]) * (t[5 * N + i] + t[6 * N + i]) +
+                                (az_y[i] + ay_z[i]) * (t[7 * N + i] + t[8 * N + i]) + 2 * ax_x[i] * t[i] +
+                                2 * ay_y[i] * t[N + i] + 2 * az_z[i] * t[2 * N + i]);
+        } else {
+            T L = M.lam(i), U = M.mu(i), E = M.eps(i);
+            T del = ((M.c13(i) + U) * (M.c13(i) + U) - (L + U) * (L + U)) / (2 * (L + U) * (L + 2 * U));
+            T gam = (M.c66(i) - U) / (2 * U);
+            T v1 = std::sqrt(2 * (L + 2 * U) * (L + U) * del + (L + U) * (L + U));
+            T v2 = ((1 + 2 * del) * L + (1 + 3 * del) * U) / v1;
+            T v3 = ((1 + 3 * del) * L + (1 + 4 * del) * U) / v1 - 1;
+            T v4 = (ax_x[i] + ay_y[i]) * t[2 * N + i] + az_z[i] * (t[i] + t[N + i]);
+            g0[i] += w * sdt * ((1 + 2 * E) * (ax_x[i] + ay_y[i]) * (t[i] + t[N + i]) + az_z[i] * t[2 * N + i] + v2 * v4);
+            g1[i] += w * sdt * (2 * (1 + 2 * E) * (ax_x[i] * t[i] + ay_y[i] * t[N + i]) + 2 * az_z[i] * t[2 * N + i] +
+                                4 * (E - gam) * (ax_x[i] * t[N + i] + ax_x[i] * t[i]) +
+                                (az_y[i] + ay_z[i]) * (t[7 * N + i] + t[8 * N + i]) +
+                                (az_x[i] + ax_z[i]) * (t[3 * N + i] + t[4 * N + i]) +
+                                (1 + 2 * gam) * (ay_x[i] + ax_y[i]) * (t[5 * N + i] + t[6 * N + i]) + v3 * v4);
+        }
+        g2[i] += (T)1.0 / sdt * rt;
+    }
+}
+
+// The nine first derivatives D_a u_c of a 3-component field, du[a*3 + c].
+template <class T>
+void all_derivatives(const T* u, T* du, const Grid& g, T dx, T dy, T dz) {
+    const size_t N = g.n();
+    for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++)
+            d1<T>(a, false, u + (size_t)c * N, du + (size_t)(a * 3 + c) * N, g, (T)h_of(a, dx, dy, dz));
+}
+
+template <class T>
+EMod<T> emod(bool vti, const T* model, size_t N) {
+    EMod<T> M;
+    M.vti = vti;
+    for (int k = 0; k < 6; k++) M.m[k] = (vti || k < 3) ? model + (size_t)k * N : nullptr;
+    return M;
+}
+
+// ----------------------------------------------------------------------------------
 // Elastic / VTI propagation. model = {lam, mu, rho[, c13, eps, c66]}; src [ns][nsrc][nt]
 // (ns = 3 forces or 6 moment components); rcv [3][nr][nt]; grad [3][N] (glam, gmu, grho);
 // snaps [slots][3][N].
@@ -445,17 +638,12 @@ void propagate_elastic(bool vti, bool adj, const T* model, const T* src, T* rcv,
     const size_t N = g.n();
     const int nt = par.nt;
     const T dt = par.dt;
-    EMod<T> M;
-    M.vti = vti;
-    for (int k = 0; k < 6; k++) M.m[k] = (vti || k < 3) ? model + (size_t)k * N : nullptr;
+    const EMod<T> M = emod<T>(vti, model, N);
     std::vector<T> u(9 * N, 0), du(9 * N, 0), tmp((grad ? 9 : 1) * N, 0);
+    std::vector<T> du_now(grad && !par.lag ? 9 * N : 0);
     T* prev = u.data();
     T* curr = u.data() + 3 * N;
     T* next = u.data() + 6 * N;
-    // du[a*3 + c] = D_a u_c  (a: derivative axis, c: component)
-    auto DU = [&](int a, int c) { return du.data() + (size_t)(a * 3 + c) * N; };
-    const T* lam = M.m[0];
-    const T* mu = M.m[1];
     const T* rho = M.m[2];
     std::vector<T> c11b(vti ? N : 0);
     for (size_t i = 0; i < c11b.size(); i++) c11b[i] = M.c11b(i);
@@ -465,64 +653,15 @@ void propagate_elastic(bool vti, bool adj, const T* model, const T* src, T* rcv,
         for (int c = 0; c < 3; c++) prev[c * N + i] *= (T)0.5 * dt * dt / rho[i];
 
     auto grad_event = [&](int itl, const T* adjf) {
-        const T sdt = (T)par.sub * dt;
-        const T* fn = snaps + (size_t)itl * 3 * N;
-        const T* fm = itl > 0 ? snaps + (size_t)(itl - 1) * 3 * N : nullptr;
-        const T* fp = itl + 1 < n_snap_slots(nt, par.sub) ? snaps + (size_t)(itl + 1) * 3 * N : nullptr;
-        T* t = tmp.data();
-        // forward-field derivatives: ux_x, uy_y, uz_z, uz_x, ux_z, uy_x, ux_y, uz_y, uy_z
-        d1<T>(AX_X, false, fn + 0 * N, t + 0 * N, g, dx);
-        d1<T>(AX_Y, false, fn + 1 * N, t + 1 * N, g, dy);
-        d1<T>(AX_Z, false, fn + 2 * N, t + 2 * N, g, dz);
-        d1<T>(AX_X, false, fn + 2 * N, t + 3 * N, g, dx);
-        d1<T>(AX_Z, false, fn + 0 * N, t + 4 * N, g, dz);
-        d1<T>(AX_X, false, fn + 1 * N, t + 5 * N, g, dx);
-        d1<T>(AX_Y, false, fn + 0 * N, t + 6 * N, g, dy);
-        d1<T>(AX_Y, false, fn + 2 * N, t + 7 * N, g, dy);
-        d1<T>(AX_Z, false, fn + 1 * N, t + 8 * N, g, dz);
-        T* g0 = grad;
-        T* g1 = grad + N;
-        T* g2 = grad + 2 * N;
-        const int last = nt / par.sub;
-        const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
-        const T w = branch == 0 ? (T)0.5 : (T)1;
-        // adjoint derivatives are the ones computed in the previous step (lagged)
-        const T *ax_x = DU(AX_X, 0), *ay_y = DU(AX_Y, 1), *az_z = DU(AX_Z, 2);
-        const T *az_x = DU(AX_Z, 0), *ax_z = DU(AX_X, 2), *ay_x = DU(AX_Y, 0), *ax_y = DU(AX_X, 1);
-        const T *az_y = DU(AX_Z, 1), *ay_z = DU(AX_Y, 2);
-#pragma omp parallel for
-        for (long long ii = 0; ii < (long long)N; ii++) {
-            size_t i = (size_t)ii;
-            T rt = 0;
-            for (int c = 0; c < 3; c++) {
-                size_t q = c * N + i;
-                const T fpv = fp ? fp[q] : (T)0;
-                T d2t = branch == 1 ? fpv - 2 * fn[q] + fm[q] : (branch == 0 ? fpv - fn[q] : -fn[q] + fm[q]);
-                rt += adjf[q] * d2t;
-            }
-            if (!vti) {
-                g0[i] += w * sdt * (ax_x[i] + ay_y[i] + az_z[i]) * (t[i] + t[N + i] + t[2 * N + i]);
-                g1[i] += w * sdt * ((az_x[i] + ax_z[i]) * (t[3 * N + i] + t[4 * N + i]) +
-                                    (ay_x[i] + ax_y[i]) * (t[5 * N + i] + t[6 * N + i]) +
-                                    (az_y[i] + ay_z[i]) * (t[7 * N + i] + t[8 * N + i]) + 2 * ax_x[i] * t[i] +
-                                    2 * ay_y[i] * t[N + i] + 2 * az_z[i] * t[2 * N + i]);
-            } else {
-                T L = M.lam(i), U = M.mu(i), E = M.eps(i);
-                T del = ((M.c13(i) + U) * (M.c13(i) + U) - (L + U) * (L + U)) / (2 * (L + U) * (L + 2 * U));
-                T gam = (M.c66(i) - U) / (2 * U);
-                T v1 = std::sqrt(2 * (L + 2 * U) * (L + U) * del + (L + U) * (L + U));
-                T v2 = ((1 + 2 * del) * L + (1 + 3 * del) * U) / v1;
-                T v3 = ((1 + 3 * del) * L + (1 + 4 * del) * U) / v1 - 1;
-                T v4 = (ax_x[i] + ay_y[i]) * t[2 * N + i] + az_z[i] * (t[i] + t[N + i]);
-                g0[i] += w * sdt * ((1 + 2 * E) * (ax_x[i] + ay_y[i]) * (t[i] + t[N + i]) + az_z[i] * t[2 * N + i] + v2 * v4);
-                g1[i] += w * sdt * (2 * (1 + 2 * E) * (ax_x[i] * t[i] + ay_y[i] * t[N + i]) + 2 * az_z[i] * t[2 * N + i] +
-                                    4 * (E - gam) * (ax_x[i] * t[N + i] + ax_x[i] * t[i]) +
-                                    (az_y[i] + ay_z[i]) * (t[7 * N + i] + t[8 * N + i]) +
-                                    (az_x[i] + ax_z[i]) * (t[3 * N + i] + t[4 * N + i]) +
-                                    (1 + 2 * gam) * (ay_x[i] + ax_y[i]) * (t[5 * N + i] + t[6 * N + i]) + v3 * v4);
-            }
-            g2[i] += (T)1.0 / sdt * rt;
+        // lagged: du holds the derivatives of the previous step's field (computed in the
+        // volume terms of that step); lag-free: the derivatives of adjf itself
+        const T* adu = du.data();
+        if (!par.lag) {
+            all_derivatives<T>(adjf, du_now.data(), g, dx, dy, dz);
+            adu = du_now.data();
         }
+        elastic_imaging<T>(vti, M, snaps, n_snap_slots(nt, par.sub), itl, nt, par.sub, dt, adjf, adu, tmp.data(), grad,
+                           g, dx, dy, dz);
     };
 
     for (int it = 0; it < nt - 1; it++) {
@@ -532,72 +671,15 @@ void propagate_elastic(bool vti, bool adj, const T* model, const T* src, T* rcv,
             for (int c = 0; c < 3; c++) extract(ext, curr + c * N, nt, c, it, rcv, g);
         if (grad && par.sub > 0 && (nt - 1 - it) % par.sub == 0 && it != 0) grad_event((nt - 1 - it) / par.sub, curr);
 
-        // second-derivative terms: D_aa(b_{c,a}) u_c
-        for (int c = 0; c < 3; c++) {
-            for (int a = 0; a < 3; a++) {
-                const T h = (T)h_of(a, dx, dy, dz);
-                T* out = next + c * N;
-                const T* in = curr + c * N;
-                const bool add = (a != AX_X);
-                if (!vti) {
-                    d2<T>(a, add, in, out, mu, g, h, a == c ? (T)2 : (T)1);
-                } else {
-                    // VTI: x-x and y-y use c11b for the matching component, c66 across x/y
-                    if ((c == 0 && a == AX_X) || (c == 1 && a == AX_Y)) d2<T>(a, add, in, out, c11b.data(), g, h, (T)1);
-                    else if ((c == 0 && a == AX_Y) || (c == 1 && a == AX_X)) d2<T>(a, add, in, out, M.m[5], g, h, 1);
-                    else d2<T>(a, add, in, out, mu, g, h, (a == AX_Z && c == 2) ? (T)2 : (T)1);
-                }
-            }
-        }
-        // normal-stress terms: next_c += D_c( sigma_cc' )
-        d1<T>(AX_X, false, curr + 0 * N, DU(AX_X, 0), g, dx);
-        d1<T>(AX_Y, false, curr + 1 * N, DU(AX_Y, 1), g, dy);
-        d1<T>(AX_Z, false, curr + 2 * N, DU(AX_Z, 2), g, dz);
-        {
-            const T *ux = DU(AX_X, 0), *uy = DU(AX_Y, 1), *uz = DU(AX_Z, 2);
-            T* t = tmp.data();
-            if (!vti) {
-                for (size_t i = 0; i < N; i++) t[i] = lam[i] * (ux[i] + uy[i] + uz[i]);
-                d1<T>(AX_X, true, t, next + 0 * N, g, dx);
-                d1<T>(AX_Y, true, t, next + 1 * N, g, dy);
-                d1<T>(AX_Z, true, t, next + 2 * N, g, dz);
-            } else {
-                for (size_t i = 0; i < N; i++) t[i] = M.c11c(i) * ux[i] + M.c12(i) * uy[i] + M.c13(i) * uz[i];
-                d1<T>(AX_X, true, t, next + 0 * N, g, dx);
-                for (size_t i = 0; i < N; i++) t[i] = M.c11c(i) * uy[i] + M.c12(i) * ux[i] + M.c13(i) * uz[i];
-                d1<T>(AX_Y, true, t, next + 1 * N, g, dy);
-                for (size_t i = 0; i < N; i++) t[i] = M.c13(i) * (ux[i] + uy[i]) + lam[i] * uz[i];
-                d1<T>(AX_Z, true, t, next + 2 * N, g, dz);
-            }
-        }
-        // mixed terms: next_c += D_a( G * D_c u_a ), a != c.  The z-derivative of the
-        // x-component term uses dy as its spacing (we_op.cpp:678, 1339).
-        const T* gxy = vti ? M.m[5] : mu;
-        d1<T>(AX_X, false, curr + 1 * N, DU(AX_X, 1), g, dx);
-        d1<T>(AX_X, false, curr + 2 * N, DU(AX_X, 2), g, dx);
-        d1<T>(AX_Y, true, DU(AX_X, 1), next + 0 * N, g, dy, gxy);
-        d1<T>(AX_Z, true, DU(AX_X, 2), next + 0 * N, g, dy, mu);
-        d1<T>(AX_Y, false, curr + 0 * N, DU(AX_Y, 0), g, dy);
-        d1<T>(AX_Y, false, curr + 2 * N, DU(AX_Y, 2), g, dy);
-        d1<T>(AX_X, true, DU(AX_Y, 0), next + 1 * N, g, dx, gxy);
-        d1<T>(AX_Z, true, DU(AX_Y, 2), next + 1 * N, g, dz, mu);
-        d1<T>(AX_Z, false, curr + 0 * N, DU(AX_Z, 0), g, dz);
-        d1<T>(AX_Z, false, curr + 1 * N, DU(AX_Z, 1), g, dz);
-        d1<T>(AX_X, true, DU(AX_Z, 0), next + 2 * N, g, dx, mu);
-        d1<T>(AX_Y, true, DU(AX_Z, 1), next + 2 * N, g, dy, mu);
-
+        elastic_volume<T>(vti, M, c11b.data(), curr, next, du.data(), tmp.data(), g, dx, dy, dz);
         for (int c = 0; c < 3; c++) inject(inj, src, nt, c, it, next + c * N, g);
-
-        const T* cu[3] = {curr, curr + N, curr + 2 * N};
-        const T* pv[3] = {prev, prev + N, prev + 2 * N};
-        T* nx3[3] = {next, next + N, next + 2 * N};
-        for (int f = 0; f < 6; f++)
-            if (par.bc[f] == 1 || par.bc[f] == 2) esat_face<T>(f, par.bc[f], cu, pv, nx3, M, g, dx, dy, dz, dt);
+        elastic_sats<T>(M, par.bc, curr, prev, next, g, dx, dy, dz, dt);
 #pragma omp parallel for
         for (long long ii = 0; ii < (long long)N; ii++) {
             size_t i = (size_t)ii;
             for (int c = 0; c < 3; c++) next[c * N + i] = dt * dt * next[c * N + i] / rho[i] + 2 * curr[c * N + i] - prev[c * N + i];
         }
+        T* nx3[3] = {next, next + N, next + 2 * N};
         escale<T>(nx3, M, g, dx, dy, dz, dt, par.bc);
         taper<T>(curr, 3, g, par.taper, par.taper_strength);
         taper<T>(next, 3, g, par.taper, par.taper_strength);

@@ -127,7 +127,10 @@ def test_acoustic_gradient_matches_oracle(eng, sub):
         assert e <= GRAD_TOL
 
 
-@pytest.mark.parametrize("shape,case", [((19, 17, 45), "mixed"), ((19, 26, 70), "free+absorb+taper")])
+@pytest.mark.parametrize("shape,case", [((19, 17, 45), "mixed"), ((19, 26, 70), "free+absorb+taper"),
+                                        # an axis of exactly 12 points: no interior tiles, so the
+                                        # interior/shell/receiver single launch gives way to the shell kernel
+                                        ((40, 12, 30), "mixed"), ((40, 12, 12), "mixed")])
 def test_acoustic_every_point_matches_oracle(eng, shape, case):
     """Receivers on every grid node (quadratic extraction weights reduce to the node
     value): the whole pressure field at every step against the oracle, through the

@@ -48,6 +48,8 @@ SHOTS = [
     ("acoustic", (36, 30, 70), 75, 4),
     ("elastic", (20, 27, 66), 110, 5),
     ("vti", (20, 27, 66), 80, 6),
+    ("elastic", (40, 12, 30), 70, 7),   # an axis of 12 points: no interior tiles
+    ("acoustic", (40, 12, 12), 60, 8),
 ]
 
 

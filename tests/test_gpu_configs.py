@@ -1,0 +1,261 @@
+"""The benchmark configurations at their real grid sizes, with a reduced number of time
+steps, against the oracle (FP32 and FP64 restatements of the reference's propagate /
+apply_forward / apply_jacobianT / compute_res_and_grad; src/cpp/we_op.cpp:567-1827,
+src/cpp/optimization.cpp:35-188) on the same inputs.
+
+* elastic shot gradient, 200^3 (5 layers + Gaussian anomaly, explosive 12 Hz source,
+  40x40 3-C receivers, FP32 snapshots every 4 steps): residual and misfit <= 1e-4,
+  gradient <= 1e-3 relative L2 against the FP32 oracle; the error against the FP64
+  oracle stays within the same bounds.
+* batched VTI modelling, one shot on each of the grids 128^3, 160x160x200 and
+  96x192x192 in one batch: traces <= 1e-4 against the FP32 and the FP64 oracle.
+* mixed-precision elastic shot gradient, 256^3, with FP32, BF16 and FP16 snapshots:
+  every gradient within 1e-3 relative L2 of the FP64 oracle; residual <= 1e-4.
+* a served mix of acoustic modelling (100^3), an elastic shot gradient (128^3) and VTI
+  modelling (160^3), submitted while the others run: every result against the FP32
+  oracle (data <= 1e-4, gradient <= 1e-3).
+
+Each oracle run takes tens of seconds on the host's cores; the numbers of time steps
+are chosen so that every test finishes in about two minutes."""
+import numpy as np
+import pytest
+
+import fwi3d_amd as F
+from fwi3d_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TRACE_TOL = 1e-4
+GRAD_TOL = 1e-3
+H = 0.01
+ZERO = (0.0, 0.0, 0.0)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = F.Engine(0)
+    yield e
+    e.close()
+
+
+def _dev():
+    import torch
+    return torch.device("cuda:0")
+
+
+def _gpu_fwi_shot(eng, kw, geo, raw, src, obs, ng):
+    """fwi3d_fwi_shot_device on device copies: (misfit, residual, FP64 gradient)."""
+    import torch
+    dev = _dev()
+    m = torch.from_numpy(np.ascontiguousarray(raw)).to(dev)
+    s = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
+    o = torch.from_numpy(np.ascontiguousarray(obs)).to(dev)
+    g = torch.zeros((ng,) + tuple(geo.shape), dtype=torch.float64, device=dev)
+    r = torch.zeros_like(o)
+    torch.cuda.synchronize()
+    mis = F.fwi_shot_device(eng, F.op_param(**kw), geo, m.data_ptr(), s.data_ptr(), o.data_ptr(), g.data_ptr(),
+                            r.data_ptr())
+    torch.cuda.synchronize()
+    return mis, r.cpu().numpy(), g.cpu().numpy()
+
+
+def _gpu_forward(eng, kw, geo, raw, src, nrc):
+    import torch
+    dev = _dev()
+    m = torch.from_numpy(np.ascontiguousarray(raw)).to(dev)
+    s = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
+    d = torch.zeros((nrc, geo.nrcv, geo.nt_data), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    F.apply_forward_device(eng, F.op_param(**kw), geo, False, m.data_ptr(), s.data_ptr(), d.data_ptr())
+    torch.cuda.synchronize()
+    return d.cpu().numpy()
+
+
+def _elastic_case(n, nt, sub, freq, delay):
+    """Layered model with the Gaussian anomaly (the elastic-gradient configuration),
+    explosive source at z = 30 m, 3-C receivers at z = 20 m on a 50 m grid; the wavelet
+    is sampled at the propagation step, so the data axis is the propagation axis."""
+    sh = W.config_b(n=n, nt=nt, sub=sub)
+    dt = sh.dt
+    w = W.ricker(freq, nt, dt, delay=delay)
+    src = np.zeros((6, nt), np.float32)
+    src[0] = src[1] = src[2] = w
+    kw = dict(nmodels=3, mt=1, dt=dt, sub=sub, bc=sh.bc, taper=sh.taper, taper_strength=sh.taper_strength)
+    sx = sh.src_loc[0]
+    geo = F.Shot(sh.shape, sh.d, ZERO, nt, dt, sx, sh.rcv_loc)
+    return sh, src, kw, geo
+
+
+def test_elastic_gradient_200cube(eng):
+    """Elastic shot gradient at 200^3 (nt = 160, sub = 4): GPU vs FP32 and FP64 oracle."""
+    nt, sub = 160, 4
+    sh, src, kw, geo = _elastic_case(200, nt, sub, 12.0, 0.08)
+    raw = sh.raw.astype(np.float32)
+    # observed data: the model without the anomaly scaled 3% slower (a residual of the
+    # whole wavefield, as the anomaly itself lies out of reach of 160 steps)
+    obs_raw = W.layered_elastic(200, H, anomaly=False)
+    obs_raw = np.stack([obs_raw[0] * np.float32(0.97), obs_raw[1] * np.float32(0.97), obs_raw[2]])
+    say(f"elastic 200^3 nt={nt}: GPU observed data, then the shot gradient")
+    obs = _gpu_forward(eng, kw, geo, obs_raw, src, 3)
+    mis, res, grad = _gpu_fwi_shot(eng, kw, geo, raw, src, obs, 3)
+    for s in ("f", "d"):
+        say(f"  oracle FP{32 if s == 'f' else 64} shot gradient")
+        g = O.ShotGeom(sh.shape, sh.d, ZERO, nt, sh.dt, sh.src_loc[0], sh.rcv_loc, s=s)
+        T = np.float32 if s == "f" else np.float64
+        omis, ores, ograd = O.fwi_shot(O.op_par(**kw), g, raw.astype(T), src.astype(T), obs.astype(T))
+        er = rel(res, ores)
+        em = abs(mis - omis) / abs(omis)
+        eg = [rel(grad[k], ograd[k]) for k in range(3)]
+        say(f"    residual {er:.2e}  misfit {em:.2e}  gradient (lambda/vp, mu/vs, rho) "
+            + " ".join(f"{e:.2e}" for e in eg))
+        assert er <= TRACE_TOL and em <= TRACE_TOL
+        assert max(eg) <= GRAD_TOL
+
+
+def test_batched_vti_three_grids(eng):
+    """One VTI shot on each batched-configuration grid, all three in one batch (nt = 100)."""
+    import torch
+    dev = _dev()
+    nt = 100
+    shots = [W.config_c_shot(g, 0, nt=nt) for g in range(3)]
+    b = F.Batch(eng)
+    bufs = []
+    for sh in shots:
+        inj = F.Injector(sh.src_kind, sh.src_loc, sh.shape, sh.d)
+        ext = F.Injector(0, sh.rcv_loc, sh.shape, sh.d)
+        par = F.make_param(nt, sh.dt, sub=0, bc=sh.bc, taper=sh.taper, taper_strength=sh.taper_strength)
+        m = torch.from_numpy(sh.model).to(dev)
+        s = torch.from_numpy(sh.src).to(dev)
+        r = torch.zeros((3, ext.npts, nt), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        sid = b.add("vti", m.data_ptr(), s.data_ptr(), r.data_ptr(), inj, ext, par, sh.shape, sh.d)
+        bufs.append((sid, m, s, r))
+    b.run()
+    torch.cuda.synchronize()
+    for sh, (sid, _, _, r) in zip(shots, bufs):
+        assert b.query(sid) == 1
+        got = r.cpu().numpy()
+        for s in ("f", "d"):
+            T = np.float32 if s == "f" else np.float64
+            oinj = O.Injector(sh.src_kind, sh.src_loc, sh.shape, sh.d, ZERO, s=s)
+            oext = O.Injector(0, sh.rcv_loc, sh.shape, sh.d, ZERO, s=s)
+            par = O.make_par(nt, 0, sh.dt, bc=sh.bc, taper=sh.taper, taper_strength=sh.taper_strength)
+            want, _ = O.propagate("vti", sh.model.astype(T), sh.src.astype(T), oinj, oext, par, sh.d, s=s)
+            e = rel(got, want)
+            say(f"VTI {sh.meta['name']} nt={nt}: traces vs FP{32 if s == 'f' else 64} oracle {e:.2e}")
+            assert e <= TRACE_TOL
+    b.close()
+
+
+def test_mixed_precision_gradient_256cube(eng):
+    """Elastic shot gradient at 256^3 (nt = 100, sub = 4) with FP32, BF16 and FP16
+    snapshots against the FP64 oracle."""
+    nt, sub = 100, 4
+    sh, src, kw, geo = _elastic_case(256, nt, sub, 20.0, 0.05)
+    raw = sh.raw.astype(np.float32)
+    obs_raw = W.layered_elastic(256, H, anomaly=False)
+    obs_raw = np.stack([obs_raw[0] * np.float32(0.97), obs_raw[1] * np.float32(0.97), obs_raw[2]])
+    say(f"elastic 256^3 nt={nt}: GPU observed data, then shot gradients with f32 / bf16 / f16 snapshots")
+    obs = _gpu_forward(eng, kw, geo, obs_raw, src, 3)
+    runs = {p: _gpu_fwi_shot(eng, dict(kw, snap_precision=p), geo, raw, src, obs, 3) for p in ("f32", "bf16", "f16")}
+    say("  oracle FP64 shot gradient")
+    g = O.ShotGeom(sh.shape, sh.d, ZERO, nt, sh.dt, sh.src_loc[0], sh.rcv_loc, s="d")
+    omis, ores, ograd = O.fwi_shot(O.op_par(**kw), g, raw.astype(np.float64), src.astype(np.float64),
+                                   obs.astype(np.float64))
+    for p, (mis, res, grad) in runs.items():
+        er = rel(res, ores)
+        em = abs(mis - omis) / abs(omis)
+        eg = [rel(grad[k], ograd[k]) for k in range(3)]
+        say(f"    {p}: residual {er:.2e}  misfit {em:.2e}  gradient (vp, vs, rho) " + " ".join(f"{e:.2e}" for e in eg))
+        assert er <= TRACE_TOL and em <= TRACE_TOL
+        assert max(eg) <= GRAD_TOL
+
+
+def _serve_request(k, kind, physics, n, nt, dt, sub):
+    """Model, wavelet and geometry of one served request (models as in the acoustic,
+    elastic and VTI configurations)."""
+    shape = (n, n, n)
+    if physics == "acoustic":
+        vz = 1.5 + 1.0 * np.arange(n) / (n - 1)
+        vp = np.broadcast_to(vz, shape).astype(np.float32)
+        raw, nm, ns = np.stack([vp, W.gardner(vp)]), 2, 1
+    elif physics == "elastic":
+        raw, nm, ns = np.stack(W.layered_elastic(n, H, anomaly=True)), 3, 6
+    else:
+        raw, nm, ns = np.stack(W.vti_model(shape, H, seed=k)), 6, 6
+    w = W.ricker(20.0, nt, dt, delay=0.05)
+    src = np.zeros((ns, nt), np.float32)
+    src[: min(ns, 3)] = w
+    c = (n - 1) * H / 2
+    rcv = W._surface_grid(c, c, 0.02, 8, 0.05)
+    sxyz = (c + 0.03 * k, c - 0.02 * k, 0.04)
+    kw = dict(nmodels=nm, mt=1 if ns == 6 else 0, dt=dt, sub=sub, bc=(1, 2, 2, 2, 2, 2), taper=(0, 15, 15, 15, 15, 15))
+    return dict(kind=kind, physics=physics, shape=shape, raw=raw.astype(np.float32), src=src, kw=kw, sxyz=sxyz,
+                rcv=rcv, nt=nt, dt=dt, nrc=1 if nm == 2 else 3, ng=min(3, nm))
+
+
+def test_served_mix_against_oracle(eng):
+    """Three requests of the serving mix on grids of 100^3 and larger, the later ones
+    submitted while the first runs; every result against the FP32 oracle."""
+    import time
+
+    import torch
+    dev = _dev()
+    reqs = [_serve_request(0, "model", "acoustic", 100, 300, 0.001, 0),
+            _serve_request(1, "gradient", "elastic", 128, 120, 0.0008, 4),
+            _serve_request(2, "model", "vti", 160, 100, 0.0008, 0)]
+    srv = F.Server(eng, hbm_budget=64 << 30, max_active=3)
+    ids, keep = [], []
+    for k, q in enumerate(reqs):
+        geo = F.Shot(q["shape"], (H,) * 3, ZERO, q["nt"], q["dt"], q["sxyz"], q["rcv"])
+        m = torch.from_numpy(q["raw"]).to(dev)
+        s = torch.from_numpy(q["src"]).to(dev)
+        d = torch.zeros((q["nrc"], geo.nrcv, q["nt"]), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        par = F.op_param(**q["kw"])
+        if q["kind"] == "model":
+            rid = srv.submit(F.REQ_MODEL, par, geo, m.data_ptr(), s.data_ptr(), data_ptr=d.data_ptr())
+            keep.append((geo, m, s, d, None, None))
+        else:
+            obs = np.zeros((q["nrc"], geo.nrcv, q["nt"]), np.float32)
+            o = torch.from_numpy(obs).to(dev)
+            g = torch.zeros((q["ng"],) + q["shape"], dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            rid = srv.submit(F.REQ_GRADIENT, par, geo, m.data_ptr(), s.data_ptr(), data_ptr=d.data_ptr(),
+                             obs_ptr=o.data_ptr(), grad64_ptr=g.data_ptr())
+            keep.append((geo, m, s, d, o, g))
+        ids.append(rid)
+        if k == 0:  # the others arrive while the first is on the device
+            t0 = time.time()
+            while srv.poll(rid)[0] == 0 and time.time() - t0 < 60:
+                time.sleep(0.001)
+    srv.wait()
+    misfits = [srv.poll(r)[1] for r in ids]
+    srv.close()
+    torch.cuda.synchronize()
+    for q, (geo, m, s, d, o, g), mis in zip(reqs, keep, misfits):
+        og = O.ShotGeom(q["shape"], (H,) * 3, ZERO, q["nt"], q["dt"], q["sxyz"], q["rcv"], s="f")
+        if q["kind"] == "model":
+            want, _ = O.apply_forward(O.op_par(**q["kw"]), og, q["raw"], q["src"])
+            e = rel(d.cpu().numpy(), want)
+            say(f"served {q['physics']} modelling {q['shape']}: data vs FP32 oracle {e:.2e}")
+            assert e <= TRACE_TOL
+        else:
+            omis, ores, ograd = O.fwi_shot(O.op_par(**q["kw"]), og, q["raw"], q["src"], o.cpu().numpy())
+            er, em = rel(d.cpu().numpy(), ores), abs(mis - omis) / abs(omis)
+            gg = g.cpu().numpy()
+            eg = [rel(gg[k], ograd[k]) for k in range(q["ng"])]
+            say(f"served {q['physics']} gradient {q['shape']}: residual {er:.2e} misfit {em:.2e} gradient "
+                + " ".join(f"{e:.2e}" for e in eg))
+            assert er <= TRACE_TOL and em <= TRACE_TOL and max(eg) <= GRAD_TOL

@@ -146,7 +146,11 @@ def test_elastic_gradient_matches_oracle(eng, physics, sub):
 
 
 @pytest.mark.parametrize("physics", ["elastic", "vti"])
-@pytest.mark.parametrize("shape,case", [((19, 17, 45), "mixed"), ((13, 26, 14), "free-all"), ((21, 14, 40), "absorb-all")])
+@pytest.mark.parametrize("shape,case", [((19, 17, 45), "mixed"), ((13, 26, 14), "free-all"), ((21, 14, 40), "absorb-all"),
+                                        # an axis of exactly 12 points: no interior tiles, and no slab
+                                        # tiles at all for (40, 12, 12) (the edge kernel covers the grid)
+                                        ((40, 12, 30), "mixed"), ((40, 12, 12), "absorb-all"),
+                                        ((12, 20, 12), "free-all")])
 def test_elastic_every_point_matches_oracle(eng, physics, shape, case):
     """Receivers on every grid node (quadratic extraction weights reduce to the node
     value), so the traces are the whole displacement field at every step: each point of
