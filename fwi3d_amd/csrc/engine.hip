@@ -569,6 +569,14 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     if (fwd_snap) c->snap_valid = true;
 }
 
+// Decode one snapshot slot (padded layout) to FP32.
+template <int FMT>
+__global__ void snap_decode(SnapView<FMT> v, float* __restrict__ out, long long n) {
+    const float sc = v.scale();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = v.code((int)i) * sc;
+}
+
 __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict__ eps, const float* __restrict__ mu,
                                long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1038,6 +1046,40 @@ int fwi3d_profile(fwi3d_ctx* c, int enable) {
         }
     }
     return FWI3D_OK;
+}
+
+int fwi3d_snapshots_read(fwi3d_ctx* c, float* out, long long count, int* slots, int* ncomp) {
+    if (!c) return FWI3D_EINVAL;
+    try {
+        REQUIRE(c->snap_slots > 0 && c->snap_valid, "no snapshot store: run a forward call with sub > 0 first");
+        if (slots) *slots = c->snap_slots;
+        if (ncomp) *ncomp = c->snap_ncomp;
+        if (!out) return FWI3D_OK;
+        const int nx = c->snap_dims[0], ny = c->snap_dims[1], nz = c->snap_dims[2];
+        const long long nunp = (long long)nx * ny * nz;
+        REQUIRE(count >= (long long)c->snap_slots * c->snap_ncomp * nunp, "output array too small for the snapshot store");
+        HIP_OK(hipSetDevice(c->device));
+        const SnapDesc S = ctx_snap(c);
+        const long long ne = (long long)S.slot_elems();
+        const int pz = (int)(S.n / ((long long)nx * ny));  // padded z pitch of the store's fields
+        c->tmpio.ensure((size_t)ne * sizeof(float));
+        float* tmp = c->tmpio.as<float>();
+        for (int k = 0; k < S.slots; k++) {
+            with_snap_fmt(S.fmt, [&](auto tag) {
+                constexpr int FMT = decltype(tag)::value;
+                hipLaunchKernelGGL(snap_decode<FMT>, dim3(2048), dim3(256), 0, c->stream, snap_view<FMT>(S, k), tmp, ne);
+            });
+            HIP_OK(hipGetLastError());
+            for (int q = 0; q < S.ncomp; q++)
+                HIP_OK(hipMemcpy2DAsync(out + ((size_t)k * S.ncomp + q) * nunp, nz * sizeof(float), tmp + (size_t)q * S.n,
+                                        pz * sizeof(float), nz * sizeof(float), (size_t)nx * ny, hipMemcpyDeviceToHost,
+                                        c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+        }
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        return fail(c, e);
+    }
 }
 
 int fwi3d_profile_read(const fwi3d_ctx* c, int kernel, double* ms, long long* n) {

@@ -120,6 +120,13 @@ int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
  * returns it. */
 int fwi3d_autotune(fwi3d_ctx *ctx, int physics, int nx, int ny, int nz, int steps, int *tx, int *ychunk);
 
+/* The snapshot store of the last forward call with sub > 0 on the context, decoded to
+ * FP32: out [slots][ncomp][ny][nx][nz] (slots = 1 + nt/sub, slot k = step k*sub), the
+ * full wavefield of zfpWrapper::getFullWfld (src/include/zfpWrapper.hpp:118-128) that
+ * WE_MODELING_3D.x writes with wavefield= (src/main/we_modeling.cpp:89-94). count: floats
+ * available at out; out NULL: only *slots and *ncomp are returned. */
+int fwi3d_snapshots_read(fwi3d_ctx *ctx, float *out, long long count, int *slots, int *ncomp);
+
 int fwi3d_profile(fwi3d_ctx *ctx, int enable);
 int fwi3d_profile_read(const fwi3d_ctx *ctx, int kernel, double *ms, long long *launches);
 
