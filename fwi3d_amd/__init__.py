@@ -31,7 +31,8 @@ EXPORTS = (
     "fwi3d_batch_query", "fwi3d_batch_wait", "fwi3d_batch_status", "fwi3d_batch_autotune", "fwi3d_batch_tiles",
     "fwi3d_batch_snap_footprint", "fwi3d_batch_add_adjoint", "fwi3d_batch_release", "fwi3d_batch_retired",
     "fwi3d_server_create", "fwi3d_server_submit", "fwi3d_server_poll", "fwi3d_server_wait", "fwi3d_server_error",
-    "fwi3d_server_result", "fwi3d_server_stats", "fwi3d_server_destroy",
+    "fwi3d_server_result", "fwi3d_server_stats", "fwi3d_server_destroy", "fwi3d_dipole_tables",
+    "fwi3d_snapshots_read",
 )
 
 
@@ -60,7 +61,7 @@ class OpParam(C.Structure):
                 ("seismotype", C.c_int), ("soft_clip", C.c_int)] + \
                [(k, C.c_float) for k in ("vpmin", "vpmax", "vsmin", "vsmax", "rhomin", "rhomax", "deltamin",
                                          "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")] + \
-               [("snap_precision", C.c_int)]
+               [("snap_precision", C.c_int), ("gl", C.c_float)]
 
 
 class ShotStruct(C.Structure):
@@ -68,7 +69,7 @@ class ShotStruct(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
                [(k, C.c_float) for k in ("dx", "dy", "dz", "ox", "oy", "oz")] + \
                [("nt_data", C.c_int), ("dt_data", C.c_float), ("sxyz", C.c_void_p), ("nrcv", C.c_int),
-                ("rxyz", C.c_void_p)]
+                ("rxyz", C.c_void_p), ("rdipaz", C.c_void_p)]
 
 
 class LbfgsParam(C.Structure):
@@ -146,6 +147,8 @@ def lib():
         L.fwi3d_server_result.argtypes = [C.c_void_p, C.c_int, C.POINTER(OpParam)]
         L.fwi3d_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerCounters)]
         L.fwi3d_server_destroy.argtypes = [C.c_void_p]
+        L.fwi3d_snapshots_read.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -181,6 +184,29 @@ class Injector:
             C.c_void_p(self.xw.ctypes.data), C.c_void_p(self.yw.ctypes.data), C.c_void_p(self.zw.ctypes.data))
         if rc != 0:
             raise ValueError("point outside the grid or invalid injector arguments")
+
+    @classmethod
+    def dipole(cls, loc5, gl, shape, d, o=(0.0, 0.0, 0.0)):
+        """DAS channels (dipole_m3, kind 2; fwi3d_dipole_tables): loc5 [npts][5] = x, y, z,
+        dip, azimuth; gauge length gl. Indices [npts][6], weights [npts][2][6]."""
+        ny, nx, nz = shape
+        loc5 = np.ascontiguousarray(np.asarray(loc5, dtype=np.float32).reshape(-1, 5))
+        self = cls.__new__(cls)
+        self.npts, self.kind = int(loc5.shape[0]), 2
+        self.xind = np.zeros((self.npts, 6), np.int32)
+        self.yind = np.zeros_like(self.xind)
+        self.zind = np.zeros_like(self.xind)
+        self.xw = np.zeros((self.npts, 2, 6), np.float32)
+        self.yw = np.zeros_like(self.xw)
+        self.zw = np.zeros_like(self.xw)
+        rc = lib().fwi3d_dipole_tables(
+            C.c_void_p(loc5.ctypes.data), self.npts, C.c_float(gl), nx, ny, nz,
+            C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]), C.c_float(o[0]), C.c_float(o[1]), C.c_float(o[2]),
+            C.c_void_p(self.xind.ctypes.data), C.c_void_p(self.yind.ctypes.data), C.c_void_p(self.zind.ctypes.data),
+            C.c_void_p(self.xw.ctypes.data), C.c_void_p(self.yw.ctypes.data), C.c_void_p(self.zw.ctypes.data))
+        if rc != 0:
+            raise ValueError("DAS channel end outside the grid or invalid arguments")
+        return self
 
     def cstruct(self):
         return InjectorStruct(self.npts, self.kind, self.xind.ctypes.data, self.yind.ctypes.data,
@@ -292,6 +318,17 @@ class Engine:
         self._check(lib().fwi3d_autotune(self._h, ph, nx, ny, nz, int(steps), C.byref(tx), C.byref(yc)),
                     "fwi3d_autotune")
         return tx.value, yc.value
+
+    def snapshots(self, shape):
+        """The snapshot store of the last forward call with sub > 0 on grid shape
+        (ny, nx, nz), decoded to FP32: [slots][ncomp][ny][nx][nz] (fwi3d_snapshots_read)."""
+        slots, ncomp = C.c_int(), C.c_int()
+        self._check(lib().fwi3d_snapshots_read(self._h, None, 0, C.byref(slots), C.byref(ncomp)),
+                    "fwi3d_snapshots_read")
+        out = np.zeros((slots.value, ncomp.value) + tuple(shape), np.float32)
+        self._check(lib().fwi3d_snapshots_read(self._h, C.c_void_p(out.ctypes.data), out.size, C.byref(slots),
+                                               C.byref(ncomp)), "fwi3d_snapshots_read")
+        return out
 
     def profile(self, enable=True, steps_only=False):
         """Start (reset) or stop HIP-event timing; steps_only records whole time steps only."""
@@ -445,15 +482,18 @@ def op_param(**kw):
 class Shot:
     """fwi3d_shot with its position arrays kept alive."""
 
-    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz):
+    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz, rdipaz=None):
+        """rdipaz [nrcv][2]: fiber dip and azimuth of DAS channels (used when gl > 0)."""
         ny, nx, nz = shape
         self.shape = tuple(shape)
         self.sxyz = np.ascontiguousarray(np.asarray(sxyz, np.float32).reshape(3))
         self.rxyz = np.ascontiguousarray(np.asarray(rxyz, np.float32).reshape(-1, 3))
+        self.rdipaz = None if rdipaz is None else np.ascontiguousarray(np.asarray(rdipaz, np.float32).reshape(-1, 2))
         self.nrcv = int(self.rxyz.shape[0])
         self.nt_data = int(nt_data)
         self.c = ShotStruct(nx, ny, nz, d[0], d[1], d[2], o[0], o[1], o[2], int(nt_data), float(dt_data),
-                            self.sxyz.ctypes.data, self.nrcv, self.rxyz.ctypes.data)
+                            self.sxyz.ctypes.data, self.nrcv, self.rxyz.ctypes.data,
+                            None if self.rdipaz is None else self.rdipaz.ctypes.data)
 
 
 def analyze_model(par, model, shot):

@@ -194,12 +194,15 @@ struct ZSlabTiles {
 };
 
 // ------------------------------------------------------------------------------------
-// Point sources / receivers. Tables: ind [npts][3], w [npts][2][nw] (row 0 inject,
-// row 1 extract). Injection runs one thread per distinct grid point touched by the
-// 27-point stencils: tbeg[ntgt+1] / tlst[27 npts] list, per target, the contributions
-// t = 27 r + 9 a + 3 b + c in increasing t, i.e. in the order of the reference's serial
-// loop (point, then y, x, z: injector.cpp:106-115), so overlapping stencils sum in a
-// fixed order and every run gives the same bits.
+// Point sources / receivers. Tables: ind [npts][npi], w [npts][2][nw] (row 0 inject,
+// row 1 extract); kinds 0 (delta_m3, npi 3, nw 3), 1 (ddelta_m3, npi 3, nw 6) and 2
+// (dipole_m3 DAS channels, npi 6, nw 6: entries 0-2 the end at -gl/2 along the fiber,
+// 3-5 the end at +gl/2). Injection runs one thread per distinct grid point touched by
+// the stencils: tbeg[ntgt+1] / tlst[nper npts] list, per target, the contributions in
+// increasing t, i.e. in the order of the reference's serial loops (point, then y, x, z,
+// and for a dipole end 1 before end 2: injector.cpp:106-115, 423-459), so overlapping
+// stencils sum in a fixed order and every run gives the same bits:
+//   kinds 0/1: t = 27 r + 9 a + 3 b + c;  kind 2: t = 54 r + 2 (9 a + 3 b + c) + e.
 struct InjTab {
     const int* xi;
     const int* yi;
@@ -212,12 +215,42 @@ struct InjTab {
     int npts, kind, nw, ntgt;
 };
 
-// Injected value of displacement/pressure component `comp` at neighbour (a,b,c) of point r.
+// Contribution t of the injection lists: point r, neighbour (a, b, c) along (y, x, z),
+// dipole end e (0 for kinds 0/1).
+__device__ __forceinline__ void inj_decode(const InjTab& I, int t, int& r, int& a, int& b, int& c, int& e) {
+    int q;
+    if (I.kind == 2) {
+        r = t / 54;
+        const int u = t - 54 * r;
+        e = u & 1;
+        q = u >> 1;
+    } else {
+        r = t / 27;
+        e = 0;
+        q = t - 27 * r;
+    }
+    a = q / 9;
+    b = (q / 3) % 3;
+    c = q % 3;
+}
+__device__ __forceinline__ void inj_point(const InjTab& I, int r, int a, int b, int c, int e, int& ix, int& iy, int& iz) {
+    const int base = I.kind == 2 ? 6 * r + 3 * e : 3 * r;
+    ix = I.xi[base + b];
+    iy = I.yi[base + a];
+    iz = I.zi[base + c];
+}
+
+// Injected value of displacement/pressure component `comp` at neighbour (a,b,c) of point
+// r (of dipole end e: -w at end 1, +w at end 2).
 __device__ __forceinline__ float inj_value(const InjTab& I, const float* __restrict__ src, int nt, int it, int comp, int r,
-                                           int a, int bb, int c) {
+                                           int a, int bb, int c, int e = 0) {
     const float* xw = I.xw + r * 2 * I.nw;
     const float* yw = I.yw + r * 2 * I.nw;
     const float* zw = I.zw + r * 2 * I.nw;
+    if (I.kind == 2) {
+        const float v = xw[3 * e + bb] * yw[3 * e + a] * zw[3 * e + c] * src[(long long)(comp * I.npts + r) * nt + it];
+        return e ? v : -v;
+    }
     if (I.kind == 0) return yw[a] * xw[bb] * zw[c] * src[(long long)(comp * I.npts + r) * nt + it];
     int icx, icy, icz;
     if (comp == 0) { icx = 0; icy = 3; icz = 4; }
@@ -237,15 +270,14 @@ __device__ __forceinline__ void acoustic_inject_one(float* __restrict__ out, con
     const float* __restrict__ K = A.K;
     const float dt = A.dt;
     const int e0 = I.tbeg[j], e1 = I.tbeg[j + 1];
-    int t = I.tlst[e0];
-    const int ix = I.xi[3 * (t / 27) + (t % 27 / 3) % 3], iy = I.yi[3 * (t / 27) + t % 27 / 9],
-              iz = I.zi[3 * (t / 27) + t % 3];
+    int r, a, bb, c, de, ix, iy, iz;
+    inj_decode(I, I.tlst[e0], r, a, bb, c, de);
+    inj_point(I, r, a, bb, c, de, ix, iy, iz);
     const int p = g.at(ix, iy, iz);
     float val = out[p];
     for (int e = e0; e < e1; e++) {
-        t = I.tlst[e];
-        const int r = t / 27, q = t % 27;
-        const float v = inj_value(I, src, nt, it, 0, r, q / 9, (q / 3) % 3, q % 3);
+        inj_decode(I, I.tlst[e], r, a, bb, c, de);
+        const float v = inj_value(I, src, nt, it, 0, r, a, bb, c, de);
         float add;
         if (mode == 1) {
             add = v * (0.5f * K[p] * dt * dt);
@@ -264,7 +296,8 @@ __global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, co
     if (j < I.ntgt) acoustic_inject_one(out, A, I, src, nt, it, mode, j);
 }
 
-// rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract);
+// rcv[comp][r][it] = Lagrange-weighted sum over the 27 points (delta_m3::extract; for a
+// dipole the difference of the two ends' sums);
 // component comp = blockIdx.y reads field u + comp * g.n.
 __device__ __forceinline__ void extract_one(const float* __restrict__ u3, const Geom& g, const InjTab& I,
                                             float* __restrict__ rcv, int nt, int it, int r, int comp) {
@@ -273,6 +306,18 @@ __device__ __forceinline__ void extract_one(const float* __restrict__ u3, const 
     const float* yw = I.yw + r * 2 * I.nw + I.nw;
     const float* zw = I.zw + r * 2 * I.nw + I.nw;
     float acc = 0.f;
+    if (I.kind == 2) {  // dipole_m3::extract (injector.cpp:461-492): u(end 2) - u(end 1)
+        const int* xi = I.xi + 6 * r;
+        const int* yi = I.yi + 6 * r;
+        const int* zi = I.zi + 6 * r;
+        for (int a = 0; a < 3; a++)
+            for (int bb = 0; bb < 3; bb++)
+                for (int c = 0; c < 3; c++)
+                    acc += xw[3 + bb] * yw[3 + a] * zw[3 + c] * u[g.at(xi[3 + bb], yi[3 + a], zi[3 + c])] -
+                           xw[bb] * yw[a] * zw[c] * u[g.at(xi[bb], yi[a], zi[c])];
+        rcv[(long long)(comp * I.npts + r) * nt + it] = acc;
+        return;
+    }
     for (int a = 0; a < 3; a++)
         for (int bb = 0; bb < 3; bb++)
             for (int c = 0; c < 3; c++)

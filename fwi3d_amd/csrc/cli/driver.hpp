@@ -152,19 +152,23 @@ inline fwi3d_op_param op_param(const RunParam& p) {
     q.vpmin = p.vpmin; q.vpmax = p.vpmax; q.vsmin = p.vsmin; q.vsmax = p.vsmax; q.rhomin = p.rhomin; q.rhomax = p.rhomax;
     q.deltamin = p.deltamin; q.deltamax = p.deltamax; q.epsilonmin = p.epsilonmin; q.epsilonmax = p.epsilonmax;
     q.gammamin = p.gammamin; q.gammamax = p.gammamax;
+    q.gl = p.gl;
     return q;
 }
 
 // One shot's geometry (positions kept alive by the caller's vectors).
 struct ShotGeo {
-    std::vector<float> sxyz, rxyz;
+    std::vector<float> sxyz, rxyz, rdipaz;
     fwi3d_shot shot{};
 };
 
 inline ShotGeo shot_geometry(const RunParam& p, const Grid3& g, int s, int nt_data, float dt_data) {
     ShotGeo o;
     o.sxyz = p.sxyz[s];
-    for (const auto& r : p.rxyz[s]) o.rxyz.insert(o.rxyz.end(), r.begin(), r.begin() + 3);
+    for (const auto& r : p.rxyz[s]) {
+        o.rxyz.insert(o.rxyz.end(), r.begin(), r.begin() + 3);
+        o.rdipaz.insert(o.rdipaz.end(), r.begin() + 3, r.begin() + 5);
+    }
     o.shot.nx = g.nx; o.shot.ny = g.ny; o.shot.nz = g.nz;
     o.shot.dx = g.dx; o.shot.dy = g.dy; o.shot.dz = g.dz;
     o.shot.ox = g.ox; o.shot.oy = g.oy; o.shot.oz = g.oz;
@@ -173,6 +177,7 @@ inline ShotGeo shot_geometry(const RunParam& p, const Grid3& g, int s, int nt_da
     o.shot.sxyz = o.sxyz.data();
     o.shot.nrcv = (int)p.rxyz[s].size();
     o.shot.rxyz = o.rxyz.data();
+    o.shot.rdipaz = o.rdipaz.data();
     return o;
 }
 

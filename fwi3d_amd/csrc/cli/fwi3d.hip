@@ -201,7 +201,6 @@ int main(int argc, char** argv) {
     unsupported(par.scale_source_times > 0, "scale_source_times");
     unsupported(par.nlsolver != "lbfgs", ("nlsolver=" + par.nlsolver).c_str());
     unsupported(par.lsearch != "regular_wolfe", ("lsearch=" + par.lsearch).c_str());
-    unsupported(par.gl > 0, "gl (DAS receivers)");
     if (ioutput_file != "none") std::fprintf(stderr, "WARNING: ioutput= (intermediate models) is not written by this build\n");
 
     P.nt = allsrc.axes[0].n;

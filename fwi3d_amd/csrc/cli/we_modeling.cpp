@@ -75,15 +75,25 @@ int main(int argc, char** argv) {
     const size_t src_per_shot = (size_t)par.nscomp * nt;
 
     const bool want_wavefield = wavefield_file != "none" && par.sub > 0 && par.ns == 1;
-    if (want_wavefield) {
-        // one shot through apply_forward, then its snapshot store
-        ShotGeo geo = shot_geometry(par, g, 0, nt, T.d);
-        fwi3d_op_param q = base;
+    fwi3d_op_param q = base;  // after apply_forward: the time axis of the last shot
+    if (want_wavefield || par.gl > 0) {
+        // one shot after the other through apply_forward (DAS channels are not served);
+        // with wavefield= (one shot) its snapshot store is written too
         DevArray ds(src_per_shot * sizeof(float)), dd(std::max<long>(1, ntr_total) * nt * sizeof(float));
-        ds.upload(allsrc.v.data(), src_per_shot * sizeof(float));
-        abi_check(fwi3d_apply_forward_device(c, &q, &geo.shot, 0, dmodel.as<float>(), ds.as<float>(), dd.as<float>()),
-                  c, "apply_forward");
-        dd.download(out.v.data(), out.v.size() * sizeof(float));
+        for (int s = 0; s < par.ns; s++) {
+            if (verbose > 1) std::fprintf(stderr, "\nStart modeling shot %d\n", s);
+            ShotGeo geo = shot_geometry(par, g, s, nt, T.d);
+            q = base;
+            if (!want_wavefield) q.sub = 0;
+            ds.upload(allsrc.v.data() + s * src_per_shot, src_per_shot * sizeof(float));
+            abi_check(fwi3d_apply_forward_device(c, &q, &geo.shot, 0, dmodel.as<float>(), ds.as<float>(),
+                                                 dd.as<float>()),
+                      c, "apply_forward");
+            const size_t ntr = (size_t)par.rxyz[s].size() * par.nrcomp;
+            dd.download(out.v.data() + ntr_cumul[s] * nt, ntr * nt * sizeof(float));
+        }
+    }
+    if (want_wavefield) {
         int slots = 0, ncomp = 0;
         abi_check(fwi3d_snapshots_read(c, nullptr, 0, &slots, &ncomp), c, "snapshots");
         Cube wf;
@@ -96,7 +106,7 @@ int main(int argc, char** argv) {
         } catch (const std::exception& e) {
             die(e.what());
         }
-    } else {
+    } else if (!(par.gl > 0)) {
         // every shot a modelling request of the server (continuous batching)
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");

@@ -81,6 +81,7 @@ struct fwi3d_ctx {
     // shot operator (operator.hip): converted model, reduction scalars, traces at the
     // propagation rate, raw FP64 gradient, synthetic data + misfit of an FWI shot
     fwi3d::DevBuf opmodel, opscal, optrace, opgrad, opsyn;
+    fwi3d::DevBuf opdas;  // per-channel fiber trigonometry of DAS receivers
     // l-BFGS (lbfgs.hip): curvature pairs, directions, backups, reduction scratch
     fwi3d::DevBuf opt;
     // snapshot-store state (the per-instance state of the reference operator)

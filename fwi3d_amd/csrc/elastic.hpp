@@ -958,15 +958,14 @@ __device__ __forceinline__ void elastic_inject_one(float* __restrict__ out3, con
     const Geom& g = A.g;
     const float dt = A.dt;
     const int e0 = I.tbeg[j], e1 = I.tbeg[j + 1];
-    int t = I.tlst[e0];
-    const int ix = I.xi[3 * (t / 27) + (t % 27 / 3) % 3], iy = I.yi[3 * (t / 27) + t % 27 / 9],
-              iz = I.zi[3 * (t / 27) + t % 3];
+    int r, a, bb, c, de, ix, iy, iz;
+    inj_decode(I, I.tlst[e0], r, a, bb, c, de);
+    inj_point(I, r, a, bb, c, de, ix, iy, iz);
     const int p = g.at(ix, iy, iz);
     float val = out[p];
     for (int e = e0; e < e1; e++) {
-        t = I.tlst[e];
-        const int r = t / 27, q = t % 27;
-        const float v = inj_value(I, src, nt, it, comp, r, q / 9, (q / 3) % 3, q % 3);
+        inj_decode(I, I.tlst[e], r, a, bb, c, de);
+        const float v = inj_value(I, src, nt, it, comp, r, a, bb, c, de);
         float add;
         if (mode == 1) {
             add = v * (0.5f * dt * dt / A.rho[p]);

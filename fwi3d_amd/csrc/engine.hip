@@ -77,22 +77,32 @@ void to_padded_on(hipStream_t stream, float* dst, const float* src, int k, const
 }
 void to_padded(fwi3d_ctx* c, float* dst, const float* src, int k, const Geom& g) { to_padded_on(c->stream, dst, src, k, g); }
 
+// Indices per point and axis, and stencil contributions per point, of an injector kind
+// (delta_m3 / ddelta_m3: 3 and 27; dipole_m3: 6 and 54).
+inline int inj_npi(int kind) { return kind == 2 ? 6 : 3; }
+inline int inj_nper(int kind) { return kind == 2 ? 54 : 27; }
+
 // Device bytes of one injector's tables (ints, floats, then the per-target lists).
 size_t inj_bytes(const fwi3d_injector* h) {
     const size_t nw = h->kind ? 6 : 3;
-    return 3 * (size_t)3 * h->npts * sizeof(int) + 3 * (2 * nw * h->npts) * sizeof(float) +
-           (2 * (size_t)27 * h->npts + 1) * sizeof(int) + 256;
+    return 3 * (size_t)inj_npi(h->kind) * h->npts * sizeof(int) + 3 * (2 * nw * h->npts) * sizeof(float) +
+           (2 * (size_t)inj_nper(h->kind) * h->npts + 1) * sizeof(int) + 256;
 }
 
 // Per-target contribution lists of the injection (InjTab::tbeg / tlst) into hv:
-// [tbeg (ntgt+1)][tlst (27 npts)]; returns ntgt. Targets in order of first appearance,
-// each target's contributions in increasing t = 27 r + 9 a + 3 b + c.
+// [tbeg (ntgt+1)][tlst (nper npts)]; returns ntgt. Targets in order of first appearance,
+// each target's contributions in increasing t (t = 27 r + 9 a + 3 b + c, or for a dipole
+// t = 54 r + 2 (9 a + 3 b + c) + e, the reference's loop order: InjTab).
 int inj_targets(const fwi3d_injector* h, std::vector<int>& hv) {
-    const int nc = 27 * h->npts;
+    const int nper = inj_nper(h->kind), npi = inj_npi(h->kind);
+    const int nc = nper * h->npts;
     std::vector<std::pair<long long, int>> key(nc);
     for (int t = 0; t < nc; t++) {
-        const int r = t / 27, q = t % 27;
-        const long long iy = h->yind[3 * r + q / 9], ix = h->xind[3 * r + (q / 3) % 3], iz = h->zind[3 * r + q % 3];
+        const int r = t / nper;
+        int q = t - nper * r, e = 0;
+        if (h->kind == 2) { e = q & 1; q >>= 1; }
+        const int base = npi * r + 3 * e;
+        const long long iy = h->yind[base + q / 9], ix = h->xind[base + (q / 3) % 3], iz = h->zind[base + q % 3];
         key[t] = {(iy << 42) | (ix << 21) | iz, t};
     }
     std::vector<int> first(nc);
@@ -130,7 +140,7 @@ InjTab upload_inj_to(hipStream_t stream, char* base, const fwi3d_injector* h, st
     t.npts = h->npts;
     t.kind = h->kind;
     t.nw = h->kind ? 6 : 3;
-    const size_t ni = (size_t)3 * h->npts, nw = (size_t)2 * t.nw * h->npts;
+    const size_t ni = (size_t)inj_npi(h->kind) * h->npts, nw = (size_t)2 * t.nw * h->npts;
     int* xi = (int*)base;
     int* yi = xi + ni;
     int* zi = yi + ni;
@@ -163,8 +173,8 @@ InjTab upload_inj(fwi3d_ctx* c, DevBuf& buf, const fwi3d_injector* h, std::vecto
 
 void check_injector(const fwi3d_injector* h, int nx, int ny, int nz, const char* what) {
     REQUIRE(h != nullptr, std::string(what) + " injector is null");
-    REQUIRE(h->npts >= 0 && (h->kind == 0 || h->kind == 1), std::string(what) + ": bad npts/kind");
-    for (int i = 0; i < 3 * h->npts; i++) {
+    REQUIRE(h->npts >= 0 && (h->kind == 0 || h->kind == 1 || h->kind == 2), std::string(what) + ": bad npts/kind");
+    for (int i = 0; i < inj_npi(h->kind) * h->npts; i++) {
         REQUIRE(h->xind[i] >= 0 && h->xind[i] < nx && h->yind[i] >= 0 && h->yind[i] < ny && h->zind[i] >= 0 &&
                     h->zind[i] < nz,
                 std::string(what) + " point outside the grid");
@@ -792,7 +802,7 @@ int fwi3d_destroy(fwi3d_ctx* c) {
     for (auto& e : c->ev) (void)hipEventDestroy(e);
     c->fields.release(); c->model.release(); c->aux.release(); c->snaps.release(); c->snapsc.release();
     c->tapers.release(); c->injtab.release(); c->exttab.release(); c->tmpio.release();
-    c->opmodel.release(); c->opscal.release(); c->optrace.release(); c->opgrad.release(); c->opsyn.release();
+    c->opmodel.release(); c->opscal.release(); c->optrace.release(); c->opgrad.release(); c->opsyn.release(); c->opdas.release();
     c->opt.release(); c->shot1_dev.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -854,6 +864,52 @@ int fwi3d_injector_tables(int kind, const float* loc, int npts, int nx, int ny, 
                   one(loc[3 * i + 1], oy, dy, ny, yind + 3 * i, yw + 2 * nw * i) &&
                   one(loc[3 * i + 2], oz, dz, nz, zind + 3 * i, zw + 2 * nw * i);
         if (!ok) return FWI3D_EINVAL;
+    }
+    return FWI3D_OK;
+}
+
+int fwi3d_dipole_tables(const float* loc5, int npts, float gl, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float ox, float oy, float oz, int* xind, int* yind, int* zind, float* xw, float* yw, float* zw) {
+    if (npts < 0 || nx < 3 || ny < 3 || nz < 3 || !(gl > 0)) return FWI3D_EINVAL;
+    // dipole_m3::findIndicesWeights (injector.cpp:313-420): a delta_m3 stencil at each end
+    // of the gauge, x -/+ gl/2 along the fiber. The reference evaluates the end positions
+    // with the double cos/sin, so pivots and alpha come from double arithmetic, alpha is
+    // stored in float, the weights as for delta_m3.
+    const float hc[4] = {(float)kHnorm[0], (float)kHnorm[1], (float)kHnorm[2], (float)kHnorm[3]};
+    auto hw = [&](int i, int n) -> float {
+        if (i < 4) return hc[i];
+        if (i >= n - 4) return hc[n - 1 - i];
+        return 1.0f;
+    };
+    auto one = [&](double pos, float o, float d, int n, int* ind, float* w) -> bool {
+        const double rr = std::round((pos - o) / d);
+        if (!(rr >= 0.0 && rr <= (double)(n - 1))) return false;
+        int piv = (int)rr;
+        if (piv == 0) piv = 1;
+        if (piv == n - 1) piv = n - 2;
+        ind[0] = piv - 1; ind[1] = piv; ind[2] = piv + 1;
+        const float a = (float)((pos - o) / d - piv);
+        const double a2 = (double)a * (double)a;
+        w[0] = (float)((a2 - a) / (2 * d * hw(piv - 1, n)));
+        w[1] = (float)((-a2 + 1) / (d * hw(piv, n)));
+        w[2] = (float)((a2 + a) / (2 * d * hw(piv + 1, n)));
+        w[6] = (float)((a2 - a) / 2);
+        w[7] = (float)(-a2 + 1);
+        w[8] = (float)((a2 + a) / 2);
+        return true;
+    };
+    for (int i = 0; i < npts; i++) {
+        const float* L = loc5 + 5 * i;
+        const double hx = gl / 2 * std::cos((double)L[3]) * std::cos((double)L[4]);
+        const double hy = gl / 2 * std::cos((double)L[3]) * std::sin((double)L[4]);
+        const double hz = gl / 2 * std::sin((double)L[3]);
+        for (int e = 0; e < 2; e++) {
+            const double sg = e ? 1.0 : -1.0;
+            bool ok = one(L[0] + sg * hx, ox, dx, nx, xind + 6 * i + 3 * e, xw + 12 * i + 3 * e) &&
+                      one(L[1] + sg * hy, oy, dy, ny, yind + 6 * i + 3 * e, yw + 12 * i + 3 * e) &&
+                      one(L[2] + sg * hz, oz, dz, nz, zind + 6 * i + 3 * e, zw + 12 * i + 3 * e);
+            if (!ok) return FWI3D_EINVAL;
+        }
     }
     return FWI3D_OK;
 }

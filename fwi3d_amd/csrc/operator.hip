@@ -317,6 +317,31 @@ __global__ __launch_bounds__(kThreads) void gradient_finish(const double* __rest
     }
 }
 
+// dipole_to_strain (src/cpp/misc.cpp:11-42) on traces p [3][nrcv][nt]; trig [nrcv] =
+// (cos dip, sin dip, cos az, sin az). Forward: component 0 <- the fiber projection / gl,
+// components 1 and 2 <- 0; adjoint: the three components from component 0.
+__global__ void dipole_strain(float* __restrict__ p, const float4* __restrict__ trig, int nrcv, int nt, float gl,
+                              int adj) {
+    const long long n = (long long)nrcv * nt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / nt);
+        const float4 t = trig[r];  // cd, sd, ca, sa
+        float* p0 = p + i;
+        float* p1 = p + n + i;
+        float* p2 = p + 2 * n + i;
+        if (!adj) {
+            *p0 = (t.x * t.z * *p0 + t.x * t.w * *p1 + t.y * *p2) / gl;
+            *p1 = 0.f;
+            *p2 = 0.f;
+        } else {
+            const float v = *p0;
+            *p2 = t.y * v / gl;
+            *p1 = t.x * t.w * v / gl;
+            *p0 = v * (t.x * t.z / gl);
+        }
+    }
+}
+
 __global__ void zero_f64(double* p, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = 0.0;
@@ -346,7 +371,7 @@ ShotDims shot_dims(const fwi3d_op_param* par, const fwi3d_shot* s) {
     const bool ac = par->nmodels == 2;
     d.ncomp = ac ? 1 : 3;
     d.nscomp = ac ? 1 : (par->mt ? 6 : 3);
-    d.nrcomp = ac ? 1 : 3;
+    d.nrcomp = (ac || par->gl > 0) ? 1 : 3;  // DAS channels record one strain component
     d.ng = std::min(3, par->nmodels);
     d.physics = ac ? FWI3D_ACOUSTIC : (par->nmodels == 3 ? FWI3D_ELASTIC : FWI3D_VTI);
     return d;
@@ -365,6 +390,8 @@ void check_op(const fwi3d_op_param* par, const fwi3d_shot* s) {
     REQUIRE(par->sinc_half_length >= 1, "sinc_half_length must be >= 1");
     REQUIRE(par->seismotype == 0 || par->seismotype == 1, "seismotype must be 0 or 1");
     REQUIRE(par->courant > 0 || par->dt > 0, "courant must be positive");
+    REQUIRE(!(par->gl > 0) || par->nmodels == 2 || s->nrcv == 0 || s->rdipaz != nullptr,
+            "DAS channels (gl > 0) need their dip and azimuth (shot rdipaz)");
 }
 
 // The per-physics overrides of analyzeGeometry (we_op.cpp:11-18) on a copy.
@@ -372,9 +399,11 @@ fwi3d_op_param normalized(const fwi3d_op_param* in) {
     fwi3d_op_param p = *in;
     if (p.nmodels == 2) {
         p.mt = 0;
+        p.gl = 0;
         p.free_surface_stiffness = std::max(p.free_surface_stiffness, 1.0f);
         p.soft_clip = 0;
     }
+    if (!(p.gl > 0)) p.gl = 0;
     for (int f = 0; f < 6; f++) p.bc[f] = std::max(0, p.bc[f]);
     return p;
 }
@@ -495,7 +524,45 @@ struct HostInj {
         }
         t = fwi3d_injector{npts, kind, xi.data(), yi.data(), zi.data(), xw.data(), yw.data(), zw.data()};
     }
+    // DAS channels (dipole_m3) of gauge length gl at rxyz with fiber dip/azimuth rdipaz
+    void build_dipole(const float* rxyz, const float* rdipaz, int npts, float gl, const fwi3d_shot* s) {
+        xi.assign(6 * (size_t)npts, 0); yi = xi; zi = xi;
+        xw.assign(12 * (size_t)npts, 0.f); yw = xw; zw = xw;
+        if (npts > 0) {
+            std::vector<float> loc5(5 * (size_t)npts);
+            for (int r = 0; r < npts; r++) {
+                for (int k = 0; k < 3; k++) loc5[5 * r + k] = rxyz[3 * r + k];
+                loc5[5 * r + 3] = rdipaz[2 * r];
+                loc5[5 * r + 4] = rdipaz[2 * r + 1];
+            }
+            const int rc = fwi3d_dipole_tables(loc5.data(), npts, gl, s->nx, s->ny, s->nz, s->dx, s->dy, s->dz, s->ox,
+                                               s->oy, s->oz, xi.data(), yi.data(), zi.data(), xw.data(), yw.data(),
+                                               zw.data());
+            REQUIRE(rc == FWI3D_OK, "a DAS channel end lies outside the grid");
+        }
+        t = fwi3d_injector{npts, 2, xi.data(), yi.data(), zi.data(), xw.data(), yw.data(), zw.data()};
+    }
+    // the receivers of a shot: points, or DAS channels when gl > 0
+    void build_receivers(const fwi3d_op_param* par, const fwi3d_shot* s) {
+        if (par->gl > 0) build_dipole(s->rxyz, s->rdipaz, s->nrcv, par->gl, s);
+        else build(0, s->rxyz, s->nrcv, s);
+    }
 };
+
+// Per-channel (cos dip, sin dip, cos az, sin az) of DAS channels on the device, each the
+// double cosine / sine rounded to float as the reference's dipole_to_strain
+// (misc.cpp:11-42) computes them.
+const float4* das_trig(fwi3d_ctx* c, const fwi3d_shot* s) {
+    std::vector<float4> h((size_t)std::max(1, s->nrcv));
+    for (int r = 0; r < s->nrcv; r++) {
+        const double dip = s->rdipaz[2 * r], az = s->rdipaz[2 * r + 1];
+        h[r] = make_float4((float)std::cos(dip), (float)std::sin(dip), (float)std::cos(az), (float)std::sin(az));
+    }
+    c->opdas.ensure(h.size() * sizeof(float4));
+    HIP_OK(hipMemcpyAsync(c->opdas.p, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));  // h is a stack vector
+    return c->opdas.as<const float4>();
+}
 
 void propagate_or_throw(fwi3d_ctx* c, int physics, int adj, const float* model, const float* src, float* rcv,
                         const fwi3d_injector* inj, const fwi3d_injector* ext, double* g64, const fwi3d_param* p,
@@ -521,9 +588,12 @@ void forward_device(fwi3d_ctx* c, fwi3d_op_param* par, const fwi3d_shot* s, int 
                        B.srcp, Tp, d.nscomp, par->resampling, par->sinc_half_length, 1.0f, 0);
     HostInj si, ri;
     si.build(par->mt ? 1 : 0, s->sxyz, 1, s);
-    ri.build(0, s->rxyz, s->nrcv, s);
+    ri.build_receivers(par, s);
     const fwi3d_param pp = prop_param(par);
     propagate_or_throw(c, d.physics, 0, B.conv, B.srcp, B.rcvp, &si.t, &ri.t, nullptr, &pp, s);
+    if (par->gl > 0 && s->nrcv > 0)  // dipole differences -> strain along the fiber (component 0)
+        hipLaunchKernelGGL(dipole_strain, dim3(nblocks((long long)s->nrcv * nt)), dim3(kThreads), 0, c->stream, B.rcvp,
+                           das_trig(c, s), s->nrcv, nt, par->gl, 0);
     const int ntr = d.nrcomp * s->nrcv;
     const float* r = B.rcvp;
     if (par->seismotype == 1) {
@@ -565,10 +635,13 @@ void jacobianT_device(fwi3d_ctx* c, fwi3d_op_param* par, const fwi3d_shot* s, in
                            B.tmp, ntr, nt, par->dt, 1);
         asrc = B.tmp;
     }
+    if (par->gl > 0 && s->nrcv > 0)  // strain -> the three dipole components (writes all three)
+        hipLaunchKernelGGL(dipole_strain, dim3(nblocks((long long)s->nrcv * nt)), dim3(kThreads), 0, c->stream, asrc,
+                           das_trig(c, s), s->nrcv, nt, par->gl, 1);
     hipLaunchKernelGGL(zero_f64, dim3(1024), dim3(kThreads), 0, c->stream, B.g64, (long long)d.ng * d.N);
     HostInj si, ri;
     si.build(par->mt ? 1 : 0, s->sxyz, 1, s);
-    ri.build(0, s->rxyz, s->nrcv, s);
+    ri.build_receivers(par, s);
     const fwi3d_param pp = prop_param(par);
     propagate_or_throw(c, d.physics, 1, B.conv, asrc, nullptr, &ri.t, &si.t, B.g64, &pp, s);
     hipLaunchKernelGGL(gradient_finish<OutT>, dim3(nblocks(d.N)), dim3(kThreads), 0, c->stream, B.g64, B.conv, d_grad,

@@ -477,6 +477,8 @@ int fwi3d_server_submit(fwi3d_server* s, const fwi3d_request* r, int* id) {
     try {
         REQUIRE(r->kind == FWI3D_REQ_MODEL || r->kind == FWI3D_REQ_GRADIENT, "unknown request kind");
         check_op(&r->par, &r->shot);
+        REQUIRE(!(r->par.gl > 0) || r->par.nmodels == 2,
+                "DAS receivers (gl > 0) are not served; use fwi3d_apply_forward_device / fwi3d_fwi_shot_device");
         REQUIRE(r->d_model && r->d_src, "null model or source");
         if (r->kind == FWI3D_REQ_MODEL) REQUIRE(r->d_data || r->shot.nrcv == 0, "null data buffer");
         else REQUIRE(r->d_grad64 && (r->d_obs || r->shot.nrcv == 0), "null gradient or observed data");

@@ -48,11 +48,14 @@ typedef struct {
 
 /* Point injector / extractor tables: the arrays of the reference's `injector`
  * (src/include/injector.hpp:8-16) as filled by findIndicesWeights
- * (src/cpp/injector.cpp:3-80, 142-237). nw = 3 (delta_m3) or 6 (ddelta_m3). */
+ * (src/cpp/injector.cpp:3-80, 142-237, 313-420). nw = 3 (delta_m3) or 6 (ddelta_m3,
+ * dipole_m3). A dipole (DAS channel) has two 3-point stencils per axis, entries 0-2 for
+ * the end at -gl/2 along the fiber and 3-5 for the end at +gl/2; it extracts
+ * u(end 2) - u(end 1) and injects -w at end 1, +w at end 2 (injector.cpp:423-492). */
 typedef struct {
     int npts;
-    int kind;                        /* 0 delta_m3, 1 ddelta_m3 (moment tensor) */
-    const int *xind, *yind, *zind;   /* [npts][3] grid indices */
+    int kind;                        /* 0 delta_m3, 1 ddelta_m3 (moment tensor), 2 dipole_m3 (DAS) */
+    const int *xind, *yind, *zind;   /* [npts][3] grid indices ([npts][6] for kind 2) */
     const float *xw, *yw, *zw;       /* [npts][2][nw]: row 0 injection, row 1 extraction */
 } fwi3d_injector;
 
@@ -72,6 +75,14 @@ const char *fwi3d_last_error(const fwi3d_ctx *ctx);
 int fwi3d_injector_tables(int kind, const float *loc, int npts, int nx, int ny, int nz, float dx, float dy,
                           float dz, float ox, float oy, float oz, int *xind, int *yind, int *zind, float *xw,
                           float *yw, float *zw);
+
+/* Dipole tables of DAS channels (kind 2): loc5 [npts][5] = x, y, z, dip, azimuth (rad;
+ * fiber direction cos(dip)cos(az), cos(dip)sin(az), sin(dip)), gauge length gl > 0.
+ * Replaces dipole_m3::findIndicesWeights (src/cpp/injector.cpp:313-420). Outputs:
+ * ind [npts][6], w [npts][2][6]. An end outside the grid gives FWI3D_EINVAL. */
+int fwi3d_dipole_tables(const float *loc5, int npts, float gl, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float ox, float oy, float oz, int *xind, int *yind, int *zind, float *xw, float *yw,
+                        float *zw);
 
 /* One forward or adjoint propagation, host pointers. Replaces
  * nl_we_op_{a,e,vti}::propagate (src/include/we_op.hpp:93,127,163; src/cpp/we_op.cpp:
@@ -161,6 +172,8 @@ typedef struct {
     float vpmin, vpmax, vsmin, vsmax, rhomin, rhomax;
     float deltamin, deltamax, epsilonmin, epsilonmax, gammamin, gammamax;
     int snap_precision;           /* fwi3d_snap_precision */
+    float gl;                     /* DAS gauge length (default 0: point receivers); > 0 (elastic,
+                                     VTI): dipole channels, one strain component per channel */
 } fwi3d_op_param;
 
 /* Geometry of one shot: grid, one source position, receiver positions and the time
@@ -173,6 +186,8 @@ typedef struct {
     const float *sxyz;            /* [3] source x, y, z (host) */
     int nrcv;
     const float *rxyz;            /* [nrcv][3] receiver x, y, z (host) */
+    const float *rdipaz;          /* [nrcv][2] fiber dip and azimuth of DAS channels (host; read
+                                     when gl > 0, else may be NULL) */
 } fwi3d_shot;
 
 /* Fill the defaults of src/include/param.hpp:8-71. */

@@ -33,6 +33,7 @@ typedef struct {
     int mt, seismotype, soft_clip;
     double vpmin, vpmax, vsmin, vsmax, rhomin, rhomax, deltamin, deltamax, epsilonmin, epsilonmax, gammamin, gammamax;
     int lag;  // as in orc_par
+    double gl;  // DAS gauge length (> 0: dipole receivers, strain data)
 } orc_op_par;
 
 // Geometry of one shot; positions are arrays of the variant's type.
@@ -44,6 +45,7 @@ typedef struct {
     const void* sxyz;
     int nrcv;
     const void* rxyz;
+    const void* rdipaz;  // [nrcv][2] dip, azimuth of DAS channels (or NULL)
 } orc_shot;
 }
 
@@ -55,9 +57,10 @@ orc::Injector<T> to_inj(const orc_inj* s) {
     I.npts = s->npts;
     I.kind = s->kind;
     const int nw = I.nw();
-    I.xi.assign(s->xi, s->xi + 3 * s->npts);
-    I.yi.assign(s->yi, s->yi + 3 * s->npts);
-    I.zi.assign(s->zi, s->zi + 3 * s->npts);
+    const int npi = I.npi();
+    I.xi.assign(s->xi, s->xi + npi * s->npts);
+    I.yi.assign(s->yi, s->yi + npi * s->npts);
+    I.zi.assign(s->zi, s->zi + npi * s->npts);
     const T* xw = (const T*)s->xw;
     const T* yw = (const T*)s->yw;
     const T* zw = (const T*)s->zw;
@@ -110,6 +113,7 @@ orc::OpParams<T> to_op(const orc_op_par* p) {
     q.rhomin = (T)p->rhomin; q.rhomax = (T)p->rhomax; q.deltamin = (T)p->deltamin; q.deltamax = (T)p->deltamax;
     q.epsilonmin = (T)p->epsilonmin; q.epsilonmax = (T)p->epsilonmax; q.gammamin = (T)p->gammamin; q.gammamax = (T)p->gammamax;
     q.lag = p->lag;
+    q.gl = (T)p->gl;
     return q;
 }
 
@@ -118,6 +122,7 @@ void from_op(const orc::OpParams<T>& q, orc_op_par* p) {
     p->nmodels = q.nmodels; p->dt = q.dt; p->courant = q.courant; p->sub = q.sub; p->nt = q.nt;
     p->vmin = q.vmin; p->vmax = q.vmax; p->mt = q.mt; p->soft_clip = q.soft_clip;
     p->free_surface_stiffness = q.free_surface_stiffness;
+    p->gl = q.gl;
     for (int i = 0; i < 6; i++) p->bc[i] = q.bc[i];
 }
 
@@ -127,7 +132,7 @@ orc::ShotGeom<T> to_shot(const orc_shot* s) {
     G.g = orc::Grid{s->nx, s->ny, s->nz};
     G.dx = (T)s->dx; G.dy = (T)s->dy; G.dz = (T)s->dz; G.ox = (T)s->ox; G.oy = (T)s->oy; G.oz = (T)s->oz;
     G.nt_data = s->nt_data; G.dt_data = (T)s->dt_data;
-    G.sxyz = (const T*)s->sxyz; G.nrcv = s->nrcv; G.rxyz = (const T*)s->rxyz;
+    G.sxyz = (const T*)s->sxyz; G.nrcv = s->nrcv; G.rxyz = (const T*)s->rxyz; G.rdipaz = (const T*)s->rdipaz;
     return G;
 }
 
@@ -254,6 +259,21 @@ orc::ShotGeom<T> to_shot(const orc_shot* s) {
         std::copy(I.xw.begin(), I.xw.end(), xw);                                                                       \
         std::copy(I.yw.begin(), I.yw.end(), yw);                                                                       \
         std::copy(I.zw.begin(), I.zw.end(), zw);                                                                       \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_dipole_injector_##S(const T* loc5, int npts, T gl, int nx, int ny, int nz, T dx, T dy, T dz, T ox, T oy,  \
+                                T oz, int* xi, int* yi, int* zi, T* xw, T* yw, T* zw) {                                \
+        auto I = orc::make_dipole<T>(loc5, npts, gl, orc::Grid{nx, ny, nz}, dx, dy, dz, ox, oy, oz);                  \
+        std::copy(I.xi.begin(), I.xi.end(), xi);                                                                       \
+        std::copy(I.yi.begin(), I.yi.end(), yi);                                                                       \
+        std::copy(I.zi.begin(), I.zi.end(), zi);                                                                       \
+        std::copy(I.xw.begin(), I.xw.end(), xw);                                                                       \
+        std::copy(I.yw.begin(), I.yw.end(), yw);                                                                       \
+        std::copy(I.zw.begin(), I.zw.end(), zw);                                                                       \
+        return 0;                                                                                                      \
+    }                                                                                                                  \
+    int orc_dipole_to_strain_##S(int adj, T* p, const T* dipaz, int nrcv, int nt, T gl) {                              \
+        orc::dipole_to_strain<T>(adj != 0, p, dipaz, nrcv, nt, gl);                                                    \
         return 0;                                                                                                      \
     }                                                                                                                  \
     int orc_inject_##S(const orc_inj* inj, const T* src, int nt, int comp, int it, T* out, int nx, int ny, int nz) {  \

@@ -161,6 +161,29 @@ class Injector:
                                             _ptr(self.xi), _ptr(self.yi), _ptr(self.zi),
                                             _ptr(self.xw), _ptr(self.yw), _ptr(self.zw))
 
+    @classmethod
+    def dipole(cls, loc5, gl, shape, d, o, s="f"):
+        """dipole_m3 tables of DAS channels loc5 [npts][5] = (x, y, z, dip, azimuth) with
+        gauge length gl: indices [npts][6], weights [npts][2][6] (entries 0-2 the end at
+        -gl/2 along the fiber, 3-5 the end at +gl/2)."""
+        ny, nx, nz = shape
+        T = _dt(s)
+        cT = C.c_float if s == "f" else C.c_double
+        loc5 = np.ascontiguousarray(np.asarray(loc5, dtype=T).reshape(-1, 5))
+        self = cls.__new__(cls)
+        self.npts, self.kind = loc5.shape[0], 2
+        self.xi = np.zeros((self.npts, 6), np.int32)
+        self.yi = np.zeros_like(self.xi)
+        self.zi = np.zeros_like(self.xi)
+        self.xw = np.zeros((self.npts, 2, 6), T)
+        self.yw = np.zeros_like(self.xw)
+        self.zw = np.zeros_like(self.xw)
+        getattr(lib(), f"orc_dipole_injector_{s}")(_ptr(loc5), self.npts, cT(gl), nx, ny, nz, cT(d[0]), cT(d[1]),
+                                                   cT(d[2]), cT(o[0]), cT(o[1]), cT(o[2]), _ptr(self.xi),
+                                                   _ptr(self.yi), _ptr(self.zi), _ptr(self.xw), _ptr(self.yw),
+                                                   _ptr(self.zw))
+        return self
+
     def as_double(self):
         """Same stencil (indices and weights of this table) with FP64 weights, for FP64
         oracle runs on the identical discrete geometry."""
@@ -172,6 +195,34 @@ class Injector:
     def cstruct(self):
         return Inj(self.npts, self.kind, self.xi.ctypes.data, self.yi.ctypes.data, self.zi.ctypes.data,
                    self.xw.ctypes.data, self.yw.ctypes.data, self.zw.ctypes.data)
+
+
+def inject(inj, src, nt, comp, it, out, s="f"):
+    """Add source samples at step it (component comp) into out [ny][nx][nz]."""
+    ny, nx, nz = out.shape[-3:]
+    I = inj.cstruct()
+    getattr(lib(), f"orc_inject_{s}")(C.byref(I), _ptr(np.ascontiguousarray(src, dtype=_dt(s))), nt, comp, it,
+                                      _ptr(out), nx, ny, nz)
+    return out
+
+
+def extract(ext, u, nt, comp, it, rcv, s="f"):
+    """rcv[comp][r][it] += receiver values of field u [ny][nx][nz]."""
+    ny, nx, nz = u.shape[-3:]
+    E = ext.cstruct()
+    getattr(lib(), f"orc_extract_{s}")(C.byref(E), _ptr(np.ascontiguousarray(u, dtype=_dt(s))), nt, comp, it,
+                                       _ptr(rcv), nx, ny, nz)
+    return rcv
+
+
+def dipole_to_strain(adj, p, dipaz, gl, s="f"):
+    """misc.cpp dipole_to_strain on traces p [3][nrcv][nt] in place."""
+    T = _dt(s)
+    cT = C.c_float if s == "f" else C.c_double
+    assert p.dtype == T and p.flags.c_contiguous
+    dipaz = np.ascontiguousarray(np.asarray(dipaz, T).reshape(-1, 2))
+    getattr(lib(), f"orc_dipole_to_strain_{s}")(int(adj), _ptr(p), _ptr(dipaz), p.shape[1], p.shape[2], cT(gl))
+    return p
 
 
 def make_par(nt, sub, dt, bc=(1, 1, 1, 1, 1, 1), taper=(0,) * 6, taper_strength=0.05, free_surface_stiffness=1.05,
@@ -253,14 +304,14 @@ class OpPar(C.Structure):
                 ("seismotype", C.c_int), ("soft_clip", C.c_int)] + \
                [(k, C.c_double) for k in ("vpmin", "vpmax", "vsmin", "vsmax", "rhomin", "rhomax", "deltamin",
                                           "deltamax", "epsilonmin", "epsilonmax", "gammamin", "gammamax")] + \
-               [("lag", C.c_int)]
+               [("lag", C.c_int), ("gl", C.c_double)]
 
 
 class ShotC(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
                [(k, C.c_double) for k in ("dx", "dy", "dz", "ox", "oy", "oz")] + \
                [("nt_data", C.c_int), ("dt_data", C.c_double), ("sxyz", C.c_void_p), ("nrcv", C.c_int),
-                ("rxyz", C.c_void_p)]
+                ("rxyz", C.c_void_p), ("rdipaz", C.c_void_p)]
 
 
 # defaults of the reference's param (src/include/param.hpp:8-71)
@@ -268,7 +319,7 @@ OP_DEFAULTS = dict(nmodels=3, dt=-1.0, courant=0.5, sub=0, nt=0, vmin=0.1, vmax=
                    sinc_half_length=11, bc=(1,) * 6, taper=(0,) * 6, taper_strength=0.05,
                    free_surface_stiffness=1.05, mt=0, seismotype=0, soft_clip=0, vpmin=0.2, vpmax=8.0, vsmin=0.1,
                    vsmax=5.0, rhomin=0.1, rhomax=8.0, deltamin=-0.5, deltamax=1.0, epsilonmin=0.0, epsilonmax=1.0,
-                   gammamin=0.0, gammamax=1.0, lag=1)
+                   gammamin=0.0, gammamax=1.0, lag=1, gl=0.0)
 
 
 def op_par(**kw):
@@ -286,21 +337,29 @@ def op_par(**kw):
 class ShotGeom:
     """Geometry of one shot with its position arrays kept alive in the variant's type."""
 
-    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz, s="f"):
+    def __init__(self, shape, d, o, nt_data, dt_data, sxyz, rxyz, s="f", rdipaz=None):
         T = _dt(s)
         self.s = s
         self.shape = tuple(shape)
         self.sxyz = np.ascontiguousarray(np.asarray(sxyz, T).reshape(3))
         self.rxyz = np.ascontiguousarray(np.asarray(rxyz, T).reshape(-1, 3))
+        self.rdipaz = None if rdipaz is None else np.ascontiguousarray(np.asarray(rdipaz, T).reshape(-1, 2))
         ny, nx, nz = shape
         self.c = ShotC(nx, ny, nz, d[0], d[1], d[2], o[0], o[1], o[2], int(nt_data), float(dt_data),
-                       self.sxyz.ctypes.data, self.rxyz.shape[0], self.rxyz.ctypes.data)
+                       self.sxyz.ctypes.data, self.rxyz.shape[0], self.rxyz.ctypes.data,
+                       None if self.rdipaz is None else self.rdipaz.ctypes.data)
         self.nrcv = self.rxyz.shape[0]
         self.nt_data = int(nt_data)
 
 
 def _ncomp(nmodels):
+    """Wavefield (snapshot) components."""
     return 1 if nmodels == 2 else 3
+
+
+def _nrcomp(par):
+    """Data components: 1 for pressure and for DAS strain."""
+    return 1 if (par.nmodels == 2 or par.gl > 0) else 3
 
 
 def analyze(par, model, geom):
@@ -360,7 +419,7 @@ def apply_forward(par, geom, model, src, add=False, data=None):
     model = np.ascontiguousarray(model, dtype=T)
     src = np.ascontiguousarray(src, dtype=T)
     if data is None:
-        data = np.zeros((_ncomp(par.nmodels), geom.nrcv, geom.nt_data), T)
+        data = np.zeros((_nrcomp(par), geom.nrcv, geom.nt_data), T)
     snaps = _snap_buffer(par, model, geom)
     rc = getattr(lib(), f"orc_apply_forward_{geom.s}")(C.byref(par), C.byref(geom.c), int(add), _ptr(model), _ptr(src),
                                                       _ptr(data), _ptr(snaps))

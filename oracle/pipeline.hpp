@@ -29,6 +29,7 @@ struct OpParams {
     T vpmin = (T)0.2, vpmax = 8, vsmin = (T)0.1, vsmax = 5, rhomin = (T)0.1, rhomax = 8;
     T deltamin = (T)-0.5, deltamax = 1, epsilonmin = 0, epsilonmax = 1, gammamin = 0, gammamax = 1;
     int lag = 1;  // test-only: lagged (1, the reference) or lag-free (0) adjoint derivatives
+    T gl = 0;     // DAS gauge length (> 0: dipole receivers and strain data, elastic/VTI)
     // outputs of analyze_model
     int nt = 0;
     T vmax = 0, vmin = 0;
@@ -215,6 +216,7 @@ struct ShotGeom {
     const T* sxyz;  // [3]
     int nrcv;
     const T* rxyz;  // [nrcv][3]
+    const T* rdipaz = nullptr;  // [nrcv][2] fiber dip and azimuth of DAS channels (gl > 0)
 };
 
 // Per-physics overrides of analyzeGeometry (we_op.cpp:11-18).
@@ -222,6 +224,7 @@ template <class T>
 void normalize_params(OpParams<T>& p) {
     if (p.nmodels == 2) {
         p.mt = 0;
+        p.gl = 0;
         p.soft_clip = 0;
         p.free_surface_stiffness = std::max(p.free_surface_stiffness, (T)1.0);
     }
@@ -241,8 +244,51 @@ PropParams<T> prop_params(const OpParams<T>& p) {
     return q;
 }
 
+// Components of the data (nrcomp: 1 for pressure and for DAS strain) and of the
+// wavefield (and of the extracted traces before the strain conversion).
 template <class T>
-inline int n_comp(const OpParams<T>& p) { return p.nmodels == 2 ? 1 : 3; }
+inline int n_comp(const OpParams<T>& p) { return (p.nmodels == 2 || p.gl > 0) ? 1 : 3; }
+template <class T>
+inline int n_field(const OpParams<T>& p) { return p.nmodels == 2 ? 1 : 3; }
+
+// Receiver injector of a shot: delta_m3 points or, with gl > 0, dipole_m3 channels.
+template <class T>
+Injector<T> receiver_injector(const OpParams<T>& par, const ShotGeom<T>& G) {
+    if (par.gl <= 0) return make_injector<T>(0, G.rxyz, G.nrcv, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
+    if (!G.rdipaz) throw std::runtime_error("DAS channels need their dip and azimuth");
+    std::vector<T> loc5((size_t)5 * G.nrcv);
+    for (int r = 0; r < G.nrcv; r++) {
+        for (int k = 0; k < 3; k++) loc5[5 * r + k] = G.rxyz[3 * r + k];
+        loc5[5 * r + 3] = G.rdipaz[2 * r];
+        loc5[5 * r + 4] = G.rdipaz[2 * r + 1];
+    }
+    return make_dipole<T>(loc5.data(), G.nrcv, par.gl, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
+}
+
+// dipole_to_strain (src/cpp/misc.cpp:11-42) on traces p [3][nrcv][nt]: forward projects
+// the end-to-end difference on the fiber and divides by gl into component 0 (zeroing
+// 1 and 2); adjoint spreads component 0 back onto the three components.
+template <class T>
+void dipole_to_strain(bool adj, T* p, const T* dipaz, int nrcv, int nt, T gl) {
+    for (int r = 0; r < nrcv; r++) {
+        const T cd = (T)std::cos((double)dipaz[2 * r]), sd = (T)std::sin((double)dipaz[2 * r]);
+        const T ca = (T)std::cos((double)dipaz[2 * r + 1]), sa = (T)std::sin((double)dipaz[2 * r + 1]);
+        T* p0 = p + (size_t)r * nt;
+        T* p1 = p + ((size_t)nrcv + r) * nt;
+        T* p2 = p + ((size_t)2 * nrcv + r) * nt;
+        for (int it = 0; it < nt; it++) {
+            if (!adj) {
+                p0[it] = (cd * ca * p0[it] + cd * sa * p1[it] + sd * p2[it]) / gl;
+                p1[it] = 0;
+                p2[it] = 0;
+            } else {
+                p2[it] = sd * p0[it] / gl;
+                p1[it] = cd * sa * p0[it] / gl;
+                p0[it] *= cd * ca / gl;
+            }
+        }
+    }
+}
 template <class T>
 inline int n_scomp(const OpParams<T>& p) { return p.nmodels == 2 ? 1 : (p.mt ? 6 : 3); }
 
@@ -263,22 +309,25 @@ void apply_forward(OpParams<T>& par, const ShotGeom<T>& G, const T* raw_model, c
     const bool acoustic = par.nmodels == 2;
     const int nscomp = n_scomp(par);
     const int nrcomp = n_comp(par);
-    std::vector<T> s(nscomp * (size_t)nt), r((size_t)nrcomp * G.nrcv * nt, 0);
+    const int nfc = n_field(par);
+    std::vector<T> s(nscomp * (size_t)nt), r((size_t)nfc * G.nrcv * nt, 0);
     std::vector<T> srcc(src, src + (size_t)nscomp * G.nt_data);
     if (par.resampling == 1) linear_resample<T>(false, false, srcc.data(), s.data(), nscomp, G.nt_data, G.dt_data, nt, par.dt);
     else sinc_resample<T>(false, false, srcc.data(), s.data(), nscomp, G.nt_data, G.dt_data, nt, par.dt, par.sinc_half_length);
     auto inj = make_injector<T>(par.mt ? 1 : 0, G.sxyz, 1, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
-    auto ext = make_injector<T>(0, G.rxyz, G.nrcv, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
+    auto ext = receiver_injector<T>(par, G);
     std::vector<T> own;
     T* sp = nullptr;
     if (par.sub > 0) {
-        const size_t ns = (size_t)n_snap_slots(nt, par.sub) * nrcomp * N;
+        const size_t ns = (size_t)n_snap_slots(nt, par.sub) * nfc * N;
         if (snaps) { sp = snaps; std::fill(sp, sp + ns, (T)0); }
         else { own.assign(ns, (T)0); sp = own.data(); }
     }
     auto P = prop_params(par);
     if (acoustic) propagate_acoustic<T>(false, model.data(), s.data(), r.data(), inj, ext, nullptr, sp, P, G.g, G.dx, G.dy, G.dz);
     else propagate_elastic<T>(par.nmodels == 6, false, model.data(), s.data(), r.data(), inj, ext, nullptr, sp, P, G.g, G.dx, G.dy, G.dz);
+    if (par.gl > 0) dipole_to_strain<T>(false, r.data(), G.rdipaz, G.nrcv, nt, par.gl);
+    r.resize((size_t)nrcomp * G.nrcv * nt);
     if (par.seismotype == 1) {
         std::vector<T> rd(r.size());
         dt_op<T>(false, r.data(), rd.data(), nrcomp * G.nrcv, nt, par.dt);
@@ -304,26 +353,28 @@ void apply_jacobianT(OpParams<T>& par, const ShotGeom<T>& G, const T* raw_model0
     const T alpha = par.dt / G.dt_data;
     const bool acoustic = par.nmodels == 2;
     const int nrcomp = n_comp(par);
+    const int nfc = n_field(par);
     const int nm = std::min(3, par.nmodels);
     std::vector<T> temp;
     if (add) temp.assign(grad, grad + nm * N);
     std::fill(grad, grad + nm * N, (T)0);
-    std::vector<T> r((size_t)nrcomp * G.nrcv * nt, 0), dd(dat, dat + (size_t)nrcomp * G.nrcv * G.nt_data);
+    std::vector<T> r((size_t)nfc * G.nrcv * nt, 0), dd(dat, dat + (size_t)nrcomp * G.nrcv * G.nt_data);
     if (par.resampling == 1) linear_resample<T>(false, false, dd.data(), r.data(), nrcomp * G.nrcv, G.nt_data, G.dt_data, nt, par.dt);
     else sinc_resample<T>(false, false, dd.data(), r.data(), nrcomp * G.nrcv, G.nt_data, G.dt_data, nt, par.dt, par.sinc_half_length);
-    apply_ht<T>(true, r.data(), nrcomp * G.nrcv, nt, par.dt);
-    for (int i = 0; i < nrcomp * G.nrcv; i++) std::reverse(r.begin() + (size_t)i * nt, r.begin() + (size_t)(i + 1) * nt);
+    apply_ht<T>(true, r.data(), nfc * G.nrcv, nt, par.dt);
+    for (int i = 0; i < nfc * G.nrcv; i++) std::reverse(r.begin() + (size_t)i * nt, r.begin() + (size_t)(i + 1) * nt);
     const T nalpha = -alpha;
     for (auto& v : r) v *= nalpha;
     if (par.seismotype == 1) {
         std::vector<T> rd(r.size());
-        dt_op<T>(true, r.data(), rd.data(), nrcomp * G.nrcv, nt, par.dt);
+        dt_op<T>(true, r.data(), rd.data(), nfc * G.nrcv, nt, par.dt);
         r.swap(rd);
     }
+    if (par.gl > 0) dipole_to_strain<T>(true, r.data(), G.rdipaz, G.nrcv, nt, par.gl);
     auto srcinj = make_injector<T>(par.mt ? 1 : 0, G.sxyz, 1, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
-    auto rinj = make_injector<T>(0, G.rxyz, G.nrcv, G.g, G.dx, G.dy, G.dz, G.ox, G.oy, G.oz);
+    auto rinj = receiver_injector<T>(par, G);
     std::vector<T> snapc;
-    if (par.sub > 0) snapc.assign(snaps, snaps + (size_t)n_snap_slots(nt, par.sub) * nrcomp * N);
+    if (par.sub > 0) snapc.assign(snaps, snaps + (size_t)n_snap_slots(nt, par.sub) * nfc * N);
     auto P = prop_params(par);
     if (acoustic) propagate_acoustic<T>(true, model.data(), r.data(), nullptr, rinj, srcinj, grad, snapc.data(), P, G.g, G.dx, G.dy, G.dz);
     else propagate_elastic<T>(par.nmodels == 6, true, model.data(), r.data(), nullptr, rinj, srcinj, grad, snapc.data(), P, G.g, G.dx, G.dy, G.dz);

@@ -252,14 +252,60 @@ void ascale(T* next, const T* K, const T* b, const Grid& g, T dx, T dy, T dz, T 
 // ----------------------------------------------------------------------------------
 // Point injectors. kind 0 (delta_m3): 3-point H-norm discrete delta per axis.
 // kind 1 (ddelta_m3): adds derivative weights for moment-tensor sources.
-// Weight layout per point: [2][nw] with row 0 injection, row 1 extraction, nw = 3 or 6.
+// kind 2 (dipole_m3, DAS channels): two kind-0 stencils at the ends of the gauge,
+// x -/+ gl/2 along the fiber; extraction is u(end 2) - u(end 1).
+// Weight layout per point: [2][nw] with row 0 injection, row 1 extraction, nw = 3 or 6
+// (kind 2: entries 0-2 end 1, 3-5 end 2); indices [npts][npi], npi = 3 or 6 (kind 2).
 template <class T>
 struct Injector {
     int npts = 0, kind = 0;
-    std::vector<int> xi, yi, zi;    // [npts][3]
+    std::vector<int> xi, yi, zi;    // [npts][npi]
     std::vector<T> xw, yw, zw;      // [npts][2][nw]
     int nw() const { return kind ? 6 : 3; }
+    int npi() const { return kind == 2 ? 6 : 3; }
 };
+
+// Tables of dipole_m3::findIndicesWeights (src/cpp/injector.cpp:313-420): loc [npts][5]
+// (x, y, z, dip, azimuth); the fiber direction is (cos dip cos az, cos dip sin az, sin dip).
+// End positions, pivots and alpha are evaluated in double (the reference calls the
+// double cos/sin), alpha stored in T, the weights from it as for delta_m3.
+template <class T>
+Injector<T> make_dipole(const T* loc, int npts, T gl, const Grid& g, T dx, T dy, T dz, T ox, T oy, T oz) {
+    Injector<T> I;
+    I.npts = npts;
+    I.kind = 2;
+    I.xi.assign(6 * npts, 0); I.yi.assign(6 * npts, 0); I.zi.assign(6 * npts, 0);
+    I.xw.assign(12 * npts, 0); I.yw.assign(12 * npts, 0); I.zw.assign(12 * npts, 0);
+    auto H = [](int i, int n) -> T { return (T)hweight(i, n); };
+    auto axis = [&](double pos, T o, T d, int n, int* ind, T* w) {
+        // pos: end position; fills ind[0..2] and w[0..2] (inject), w[6..8] (extract)
+        int piv = (int)std::round((pos - o) / d);
+        if (piv == 0) piv = 1;
+        if (piv == n - 1) piv = n - 2;
+        ind[0] = piv - 1; ind[1] = piv; ind[2] = piv + 1;
+        const T a = (T)((pos - o) / d - piv);
+        const double a2 = std::pow(a, 2);
+        w[0] = (T)((a2 - a) / (2 * d * H(piv - 1, n)));
+        w[1] = (T)((-a2 + 1) / (d * H(piv, n)));
+        w[2] = (T)((a2 + a) / (2 * d * H(piv + 1, n)));
+        w[6] = (T)((a2 - a) / 2);
+        w[7] = (T)(-a2 + 1);
+        w[8] = (T)((a2 + a) / 2);
+    };
+    for (int i = 0; i < npts; i++) {
+        const T* L = loc + 5 * i;
+        const double hx = gl / 2 * std::cos((double)L[3]) * std::cos((double)L[4]);
+        const double hy = gl / 2 * std::cos((double)L[3]) * std::sin((double)L[4]);
+        const double hz = gl / 2 * std::sin((double)L[3]);
+        for (int e = 0; e < 2; e++) {
+            const double sg = e ? 1.0 : -1.0;
+            axis(L[0] + sg * hx, ox, dx, g.nx, &I.xi[6 * i + 3 * e], &I.xw[12 * i + 3 * e]);
+            axis(L[1] + sg * hy, oy, dy, g.ny, &I.yi[6 * i + 3 * e], &I.yw[12 * i + 3 * e]);
+            axis(L[2] + sg * hz, oz, dz, g.nz, &I.zi[6 * i + 3 * e], &I.zw[12 * i + 3 * e]);
+        }
+    }
+    return I;
+}
 
 template <class T>
 Injector<T> make_injector(int kind, const T* loc /*[npts][3] x,y,z*/, int npts, const Grid& g, T dx, T dy, T dz, T ox,
@@ -314,6 +360,22 @@ void inject(const Injector<T>& I, const T* src, int nt, int comp, int it, T* out
         else if (comp == 1) { icx = 3; icy = 1; icz = 5; }
         else { icx = 4; icy = 5; icz = 2; }
     }
+    if (I.kind == 2) {  // dipole_m3::inject (injector.cpp:423-459): -end 1, +end 2
+        for (int r = 0; r < I.npts; r++) {
+            const T* xw = &I.xw[12 * r];
+            const T* yw = &I.yw[12 * r];
+            const T* zw = &I.zw[12 * r];
+            const T s = src[((size_t)comp * I.npts + r) * nt + it];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int c = 0; c < 3; c++) {
+                        out[g.at(I.xi[6 * r + b], I.yi[6 * r + a], I.zi[6 * r + c])] -= xw[b] * yw[a] * zw[c] * s;
+                        out[g.at(I.xi[6 * r + 3 + b], I.yi[6 * r + 3 + a], I.zi[6 * r + 3 + c])] +=
+                            xw[3 + b] * yw[3 + a] * zw[3 + c] * s;
+                    }
+        }
+        return;
+    }
     for (int r = 0; r < I.npts; r++) {
         const T* xw = &I.xw[2 * nw * r];
         const T* yw = &I.yw[2 * nw * r];
@@ -339,6 +401,22 @@ void inject(const Injector<T>& I, const T* src, int nt, int comp, int it, T* out
 template <class T>
 void extract(const Injector<T>& I, const T* u, int nt, int comp, int it, T* rcv, const Grid& g) {
     const int nw = I.nw();
+    if (I.kind == 2) {  // dipole_m3::extract (injector.cpp:461-492): u(end 2) - u(end 1)
+        for (int r = 0; r < I.npts; r++) {
+            const T* xw = &I.xw[12 * r + 6];
+            const T* yw = &I.yw[12 * r + 6];
+            const T* zw = &I.zw[12 * r + 6];
+            T acc = 0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int c = 0; c < 3; c++)
+                        acc += xw[3 + b] * yw[3 + a] * zw[3 + c] *
+                                   u[g.at(I.xi[6 * r + 3 + b], I.yi[6 * r + 3 + a], I.zi[6 * r + 3 + c])] -
+                               xw[b] * yw[a] * zw[c] * u[g.at(I.xi[6 * r + b], I.yi[6 * r + a], I.zi[6 * r + c])];
+            rcv[((size_t)comp * I.npts + r) * nt + it] += acc;
+        }
+        return;
+    }
     for (int r = 0; r < I.npts; r++) {
         const T* xw = &I.xw[2 * nw * r + nw];
         const T* yw = &I.yw[2 * nw * r + nw];

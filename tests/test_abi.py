@@ -109,3 +109,25 @@ def test_host_analyze_model_matches_oracle(nmodels, dt, sub):
     assert (p.nt, p.sub) == (q.nt, q.sub)
     assert np.float32(p.dt) == np.float32(q.dt) and np.float32(p.courant) == np.float32(q.courant)
     assert np.float32(p.vmax) == np.float32(q.vmax) and np.float32(p.vmin) == np.float32(q.vmin)
+
+
+def test_dipole_tables_match_oracle():
+    """fwi3d_dipole_tables (host code) against the oracle's dipole_m3 restatement: same
+    indices and weights (pinned themselves in test_oracle_das.py)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(8)
+    shape = (30, 34, 38)
+    ny, nx, nz = shape
+    d = (0.01, 0.012, 0.009)
+    L = np.array([(nx - 1) * d[0], (ny - 1) * d[1], (nz - 1) * d[2]])
+    loc5 = np.array([list(rng.uniform(0.2, 0.8, 3) * L) + [rng.uniform(-1.5, 1.5), rng.uniform(-3, 3)]
+                     for _ in range(40)], np.float32)
+    gl = 0.03
+    a = F.Injector.dipole(loc5, gl, shape, d, (0.0, 0.0, 0.0))
+    b = O.Injector.dipole(loc5, gl, shape, d, (0, 0, 0), s="f")
+    for k in ("xind", "yind", "zind"):
+        assert np.array_equal(getattr(a, k), getattr(b, k[0] + "i")), k
+    for k in ("xw", "yw", "zw"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    with pytest.raises(ValueError):
+        F.Injector.dipole([[0.0, 0.0, 0.0, 0.0, 0.0]], gl, shape, d)  # an end falls outside the grid

@@ -214,3 +214,38 @@ def test_fwi3d_objective_and_descent(tmp_path):
     inv2 = inv2.reshape(3, ny, nx, nz)
     assert f2[-1] < f2[0]
     assert np.array_equal(inv2[:, :, :, :3], start[:, :, :, :3])  # masked cells keep their values
+
+
+def test_we_modeling_das(tmp_path):
+    """DAS channels (gl > 0): the srcoord dip / azimuth columns orient the fibers; one
+    strain trace per channel, against the oracle's apply_forward."""
+    shape = (24, 26, 22)
+    ny, nx, nz = shape
+    raw = model_raw("elastic", shape)
+    nt, dt = 100, 0.002
+    w = ricker(nt, dt)
+    write_cube(tmp_path / "model.H", raw, [(nz, 0, H), (nx, 0, H), (ny, 0, H), (3, 0, 1)])
+    write_cube(tmp_path / "wavelet.H", w, [(nt, 0, dt)])
+    shots = [(0, (0.12, 0.12, 0.08), [(0.06 + 0.03 * i, 0.10, 0.05, 0.3 * i - 0.4, 0.5 * i) for i in range(5)]),
+             (1, (0.15, 0.10, 0.09), [(0.08, 0.06 + 0.03 * i, 0.06, 0.2, 1.0 - 0.3 * i) for i in range(4)])]
+    with open(tmp_path / "geom.txt", "w") as f:
+        for sid, s, rs in shots:
+            for r in rs:
+                f.write(f"{sid} {s[0]} {s[1]} {s[2]} {r[0]} {r[1]} {r[2]} {r[3]} {r[4]}\n")
+    args = [f"srcoord={tmp_path}/geom.txt", "gl=0.02", "format=1", f"datapath={tmp_path}", "courant=0.4",
+            "bc_top=1", "bc_bottom=2", "bc_left=2", "bc_right=2", "bc_front=2", "bc_back=2",
+            "source=wavelet.H", "model=model.H", "output=das.H"]
+    run("WE_MODELING_3D.x", args, tmp_path)
+    got, n, _ = read_cube(tmp_path / "das.H")
+    assert n[:2] == [nt, 9]
+    got = got.reshape(9, nt)
+    row = 0
+    src = np.stack([w, w * 0, w * 0]).astype(np.float32)
+    for sid, s, rs in shots:
+        g = O.ShotGeom(shape, (H,) * 3, (0, 0, 0), nt, dt, s, [r[:3] for r in rs], s="f", rdipaz=[r[3:] for r in rs])
+        kw = dict(nmodels=3, gl=0.02, bc=(1, 2, 2, 2, 2, 2), courant=0.4, dt=-1, sub=0)
+        want, _ = O.apply_forward(O.op_par(**kw), g, raw, src)
+        e = rel(got[row:row + len(rs)], want.reshape(len(rs), nt))
+        print(f"DAS shot {sid}: strain traces vs oracle {e:.2e}", flush=True)
+        assert e <= 1e-4
+        row += len(rs)
