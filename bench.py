@@ -31,29 +31,51 @@ ELASTIC_BYTES_PER_CELL = 48.0  # algorithmic bytes per elastic cell-update (u, u
 ACOUSTIC_BYTES_PER_CELL = 20.0  # u, u_prev, u_next + K, 1/rho
 
 
-def cpu_baseline(shot, seconds=15.0):
-    """Time the oracle (FP32 CPU restatement, OpenMP) on a bounded sample of the same
-    workload: forward time steps of the same model and geometry."""
+def cpu_model():
+    """The host CPU's model name (lscpu / /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(shot, seconds=6.0):
+    """Time the oracle (FP32 CPU restatement of the reference's propagate and
+    compute_gradients, OpenMP) on a bounded sample of the same workload: a forward run of
+    nsteps time steps with snapshots every `sub` steps, then the adjoint run with the
+    imaging condition (forward + adjoint cell-updates, the GPU figure's unit). One
+    warm-up, then the median of 3."""
     from oracle import oracle as O
     shape = shot.shape
     inj = O.Injector(shot.src_kind, shot.src_loc, shape, shot.d, (0, 0, 0))
     ext = O.Injector(0, shot.rcv_loc, shape, shot.d, (0, 0, 0))
 
     def run(nsteps):
-        par = O.make_par(nsteps + 1, 0, shot.dt, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
-        src = np.ascontiguousarray(shot.src[..., :nsteps + 1])
+        nt = nsteps + 1
+        par = O.make_par(nt, shot.sub, shot.dt, bc=shot.bc, taper=shot.taper, taper_strength=shot.taper_strength)
+        src = np.ascontiguousarray(shot.src[..., :nt])
         t = time.perf_counter()
-        O.propagate(shot.physics, shot.model, src, inj, ext, par, shot.d)
+        rcv, snaps = O.propagate(shot.physics, shot.model, src, inj, ext, par, shot.d)
+        grad = np.zeros((3,) + tuple(shape), np.float32)
+        O.propagate(shot.physics, shot.model, np.ascontiguousarray(rcv[:, :, ::-1]), ext, inj, par, shot.d, adj=True,
+                    grad=grad, snaps=snaps)
         return time.perf_counter() - t
 
-    t2 = run(2)
-    nsteps = int(max(2, min(200, seconds / max(t2 / 2, 1e-3))))
-    t = run(nsteps)
+    t4 = run(4)  # warm-up, and the per-step cost
+    nsteps = int(max(4, min(400, seconds / max(t4 / 4, 1e-3))))
+    nsteps -= nsteps % shot.sub
+    times = sorted(run(nsteps) for _ in range(3))
+    t = times[1]
     cells = float(np.prod(shape))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{nsteps} forward time steps of the same {shape[2]}x{shape[1]}x{shape[0]} {shot.physics} shot "
-                      f"(oracle FP32 C++/OpenMP, {t:.1f} s)"}
+    return {"value": round(2.0 * cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port", "cpu": cpu_model(), "runs_s": [round(x, 2) for x in times],
+            "sample": f"{nsteps} forward time steps with snapshots every {shot.sub} steps, then {nsteps} adjoint steps "
+                      f"with the imaging condition, of the same {shape[2]}x{shape[1]}x{shape[0]} {shot.physics} shot "
+                      f"(oracle FP32 C++/OpenMP); median of 3 runs after one warm-up"}
 
 
 def measured_traffic(n):
@@ -164,7 +186,7 @@ def run_lbfgs(args, torch, F, W):
     eng.close()
 
 
-def run_batch(args, torch, F, W):
+def run_batch(args, torch, F, W, emit=True):
     """Batched VTI forward modelling: 32 shots split 11/11/10 over the grids (nz,nx,ny) =
     (128,128,128), (200,160,160), (96,192,192), nt=1500 (or --nt), dt=0.8 ms. The same
     shots run one at a time through the plain path (fwi3d_propagate_device) and then
@@ -251,8 +273,10 @@ def run_batch(args, torch, F, W):
         "batch_candidate_ms_per_step": {str(y): round(t, 3) for y, t in zip((4, 8, 16, 32), batch_tiles[1])},
         "traces_identical_to_plain": bool(same),
     }
-    print(json.dumps(out), flush=True)
     eng.close()
+    if not emit:
+        return out
+    print(json.dumps(out), flush=True)
 
 
 def run_acoustic(args, torch, F, W):
@@ -406,7 +430,7 @@ def serve_inputs(torch, F, W, req, dev):
     return out
 
 
-def run_serve(args, torch, F, W):
+def run_serve(args, torch, F, W, emit=True):
     """Serving stream: --requests seeded requests (60% acoustic modelling n^3, n in
     100..200; 30% elastic shot gradients, n in 128..256, BF16 snapshots every 4 steps;
     10% VTI modelling 160^3; nt in 1000..3000). (1) plain: one request after another
@@ -556,7 +580,51 @@ def run_serve(args, torch, F, W):
         "plain_latency_s": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)},
         "results_identical_to_plain": bool(same),
     }
+    if not emit:
+        return out
     print(json.dumps(out), flush=True)
+
+
+def config_d_gradients(torch, F, W, n=256, nt=3000):
+    """The mixed-precision gradient configuration's shot gradient at full size (256^3,
+    nt=3000, sub=4; layered model with the anomaly, anomaly-free observed data): one shot
+    gradient with FP32 snapshots and one with BF16 snapshots, each timed, and the BF16
+    gradient's relative L2 error against the FP32 one."""
+    true = W.config_b(n=n, nt=nt)
+    obs_model = W.config_b(n=n, nt=nt, anomaly=False)
+    shape = true.shape
+    dev = torch.device("cuda:0")
+    eng = F.Engine(0)
+    geo = F.Shot(shape, true.d, (0.0, 0.0, 0.0), nt, true.dt, true.src_loc[0], true.rcv_loc)
+    opkw = dict(nmodels=3, dt=true.dt, sub=true.sub, mt=1, bc=true.bc, taper=true.taper,
+                taper_strength=true.taper_strength)
+    src = torch.from_numpy(np.ascontiguousarray(true.src[:, 0, :])).to(dev)
+    m = torch.from_numpy(np.ascontiguousarray(obs_model.raw)).to(dev)
+    dobs = torch.zeros((3, geo.nrcv, nt), dtype=torch.float32, device=dev)
+    F.apply_forward_device(eng, F.op_param(**dict(opkw, sub=0)), geo, False, m.data_ptr(), src.data_ptr(),
+                           dobs.data_ptr())
+    m = torch.from_numpy(np.ascontiguousarray(true.raw)).to(dev)
+    g64 = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+    N = float(np.prod(shape))
+    res, grads = {}, {}
+    for fmt in ("f32", "bf16"):
+        g64.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = F.fwi_shot_device(eng, F.op_param(**dict(opkw, snap_precision=fmt)), geo, m.data_ptr(), src.data_ptr(),
+                              dobs.data_ptr(), g64.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        res[fmt] = {"shot_gradient_s": round(t, 3), "gcell_updates_per_s": round(2 * N * (nt - 1) / t / 1e9, 3),
+                    "misfit": f}
+        grads[fmt] = g64.clone()
+        print(f"# config D {fmt}: {t:.2f} s", file=sys.stderr, flush=True)
+    err = [float(torch.linalg.vector_norm(grads["bf16"][k] - grads["f32"][k]) / torch.linalg.vector_norm(grads["f32"][k]))
+           for k in range(3)]
+    eng.close()
+    return {"workload": f"elastic shot gradient {shape[2]}x{shape[1]}x{shape[0]}, nt={nt}, sub={true.sub}, FP32 and "
+                        f"BF16 (per-snapshot max-abs scale) snapshots, FP64 gradient accumulation",
+            **res, "bf16_vs_f32_gradient_rel_l2": {"vp": err[0], "vs": err[1], "rho": err[2]}}
 
 
 def main():
@@ -567,6 +635,9 @@ def main():
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision 256^3 gradients, the batched VTI comparison and the reduced "
+                         "serving stream that the default run adds after the headline")
     ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
     ap.add_argument("--workload", default="gradient", choices=("gradient", "acoustic", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); acoustic: 100^3 acoustic forward modelling; "
@@ -697,6 +768,10 @@ def main():
                                f"accumulation (fwi3d_fwi_shot_device)",
                    "nz": nz, "nx": nx, "ny": ny, "nt": nt, "receivers": int(geo.nrcv) * 3,
                    "autotuned_tiles_tx_ychunk": list(tiles)},
+        "ms_per_shot_gradient": round(elapsed / args.steps * 1e3, 2),
+        "time_steps_per_iteration": 2 * (nt - 1),
+        "step_definition": "one bench step is one shot gradient: forward propagation with snapshots, residual, "
+                           "adjoint propagation with the imaging condition (2 x (nt-1) time steps)",
         "shot_gradients_per_s": round(args.steps / elapsed, 4),
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
         "misfit": misfits[-1] if misfits else None,
@@ -720,10 +795,27 @@ def main():
                            "gradient with every kernel bracketed by events",
         },
     }
+    eng.close()
+    del model, model_obs, src, dobs, grad
+    torch.cuda.empty_cache()
+    if not args.no_extras and args.n == 200 and args.nt == 2000:
+        # the other configurations, untimed by the headline: mixed-precision 256^3
+        # gradients, 32 batched VTI shots against one at a time, a reduced serving stream
+        print("# extras: 256^3 gradients (FP32, BF16 snapshots)", file=sys.stderr, flush=True)
+        out["config_mixed_precision_256"] = config_d_gradients(torch, F, W)
+        torch.cuda.empty_cache()
+        print("# extras: batched VTI shots", file=sys.stderr, flush=True)
+        out["config_batched_vti"] = run_batch(argparse.Namespace(nt=1500), torch, F, W, emit=False)
+        torch.cuda.empty_cache()
+        print("# extras: serving stream (16 requests)", file=sys.stderr, flush=True)
+        sv = run_serve(argparse.Namespace(requests=16, max_active=12, hbm_budget_gib=200.0), torch, F, W, emit=False)
+        out["config_serving"] = {k: sv[k] for k in ("config", "shot_gradients_per_s", "requests_per_s",
+                                                    "speedup_vs_plain", "poisson", "plain_latency_s",
+                                                    "results_identical_to_plain")}
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(shot)
     print(json.dumps(out, default=lambda o: o.item() if hasattr(o, 'item') else str(o)), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":
