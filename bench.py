@@ -42,12 +42,13 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(shot, seconds=6.0):
+def cpu_baseline(shot, seconds=6.0, gradient=True):
     """Time the oracle (FP32 CPU restatement of the reference's propagate and
     compute_gradients, OpenMP) on a bounded sample of the same workload: a forward run of
     nsteps time steps with snapshots every `sub` steps, then the adjoint run with the
-    imaging condition (forward + adjoint cell-updates, the GPU figure's unit). One
-    warm-up, then the median of 3."""
+    imaging condition (forward + adjoint cell-updates, the GPU figure's unit); with
+    gradient=False the forward run only (modelling workloads). One warm-up, then the
+    median of 3."""
     from oracle import oracle as O
     shape = shot.shape
     inj = O.Injector(shot.src_kind, shot.src_loc, shape, shot.d, (0, 0, 0))
@@ -59,22 +60,25 @@ def cpu_baseline(shot, seconds=6.0):
         src = np.ascontiguousarray(shot.src[..., :nt])
         t = time.perf_counter()
         rcv, snaps = O.propagate(shot.physics, shot.model, src, inj, ext, par, shot.d)
-        grad = np.zeros((3,) + tuple(shape), np.float32)
-        O.propagate(shot.physics, shot.model, np.ascontiguousarray(rcv[:, :, ::-1]), ext, inj, par, shot.d, adj=True,
-                    grad=grad, snaps=snaps)
+        if gradient:
+            grad = np.zeros((3,) + tuple(shape), np.float32)
+            O.propagate(shot.physics, shot.model, np.ascontiguousarray(rcv[:, :, ::-1]), ext, inj, par, shot.d,
+                        adj=True, grad=grad, snaps=snaps)
         return time.perf_counter() - t
 
     t4 = run(4)  # warm-up, and the per-step cost
     nsteps = int(max(4, min(400, seconds / max(t4 / 4, 1e-3))))
-    nsteps -= nsteps % shot.sub
+    if shot.sub > 0:
+        nsteps -= nsteps % shot.sub
     times = sorted(run(nsteps) for _ in range(3))
     t = times[1]
     cells = float(np.prod(shape))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(2.0 * cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
-            "kind": "port", "cpu": cpu_model(), "runs_s": [round(x, 2) for x in times],
-            "sample": f"{nsteps} forward time steps with snapshots every {shot.sub} steps, then {nsteps} adjoint steps "
-                      f"with the imaging condition, of the same {shape[2]}x{shape[1]}x{shape[0]} {shot.physics} shot "
+    what = (f"{nsteps} forward time steps with snapshots every {shot.sub} steps, then {nsteps} adjoint steps with the "
+            f"imaging condition" if gradient else f"{nsteps} forward time steps")
+    return {"value": round((2.0 if gradient else 1.0) * cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s",
+            "cores": cores, "kind": "port", "cpu": cpu_model(), "runs_s": [round(x, 2) for x in times],
+            "sample": f"{what}, of the same {shape[2]}x{shape[1]}x{shape[0]} {shot.physics} shot "
                       f"(oracle FP32 C++/OpenMP); median of 3 runs after one warm-up"}
 
 
@@ -353,7 +357,7 @@ def run_acoustic(args, torch, F, W):
                      "note": "a 100^3 step's 20 MB working set is served from the 256 MB Infinity Cache"},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(shot)
+        out["cpu_baseline"] = cpu_baseline(shot, gradient=False)
     print(json.dumps(out), flush=True)
     eng.close()
 
@@ -639,6 +643,8 @@ def main():
                     help="skip the mixed-precision 256^3 gradients, the batched VTI comparison and the reduced "
                          "serving stream that the default run adds after the headline")
     ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="elastic / VTI steps with the flux, interior, slab and edge kernels instead of the fused kernel")
     ap.add_argument("--workload", default="gradient", choices=("gradient", "acoustic", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); acoustic: 100^3 acoustic forward modelling; "
                          "lbfgs: 256^3 mixed-precision l-BFGS FWI; batch: 32 VTI shots on 3 grids, batched vs one at "
@@ -683,6 +689,8 @@ def main():
     nt = shot.nt
     dev = torch.device("cuda:0")
     eng = F.Engine(0)
+    if args.no_fused:
+        eng.tune(fused=False)
     geo = F.Shot(shape, shot.d, (0.0, 0.0, 0.0), nt, shot.dt, shot.src_loc[0], shot.rcv_loc)
     # the reference's param for this shot: raw (vp, vs, rho), explosive moment tensor,
     # propagation dt = data dt (sinc resampling still runs), imaging every 4 steps

@@ -299,13 +299,19 @@ class Engine:
                                           C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device")
 
-    def tune(self, ychunk=None, tx=None, graphs=None):
-        """Tiling of the time-step kernels and graph replay of time loops (fwi3d_tune);
-        results do not depend on them."""
+    def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None):
+        """Tiling of the time-step kernels, graph replay of time loops, and the elastic /
+        VTI step form (fused: the fused kernel over the planes 6 cells inside the y faces;
+        fchunk: its y planes per workgroup, 0 by grid size) (fwi3d_tune). Results do not
+        depend on the tiling or on graph replay; the two elastic forms agree to rounding."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")
         if tx is not None:
             self._check(lib().fwi3d_tune(self._h, 1, int(tx)), "fwi3d_tune")
+        if fused is not None:
+            self._check(lib().fwi3d_tune(self._h, 3, 1 if fused else 0), "fwi3d_tune")
+        if fchunk is not None:
+            self._check(lib().fwi3d_tune(self._h, 4, int(fchunk)), "fwi3d_tune")
         if graphs is not None:
             self._check(lib().fwi3d_tune(self._h, 2, int(bool(graphs))), "fwi3d_tune")
 

@@ -46,11 +46,11 @@ __device__ __forceinline__ void acoustic_interior_tile(const AcuArgs A, const LF
     constexpr int TZ = 64;
     __shared__ float su[TX + 4][TZ + 4];
     __shared__ float sb[TX + 4][TZ + 4];
-    const float* __restrict__ curr = A.curr;
-    const float* __restrict__ prev = A.prev;
-    float* __restrict__ next = A.next;
-    const float* __restrict__ K = A.K;
-    const float* __restrict__ bb = A.b;
+    const GAS float* __restrict__ curr = gas(A.curr);
+    const GAS float* __restrict__ prev = gas(A.prev);
+    GAS float* __restrict__ next = gas(A.next);
+    const GAS float* __restrict__ K = gas(A.K);
+    const GAS float* __restrict__ bb = gas(A.b);
     const float* __restrict__ tz2 = A.tz2;
     const float* __restrict__ tx2 = A.tx2;
     const float* __restrict__ ty2 = A.ty2;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void acoustic_interior_tile(const AcuArgs A, const LF
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
     const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
-    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    auto ldy = [&](const GAS float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
     HaloMap<TX> H;
     H.init(lx, lz, x0, z0, nx, nz, sx);
 
@@ -142,14 +142,15 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b,
     int ix, iy, iz;
     const int t = b * 256 + threadIdx.y * blockDim.x + threadIdx.x;  // 256 x 1 or 64 x 4 threads
     if (t >= npts || !shell_point(t, 6, g.nx, g.ny, g.nz, ix, iy, iz)) return;
-    const float* __restrict__ curr = A.curr;
-    const float* __restrict__ bb = A.b;
-    const float* __restrict__ K = A.K;
+    const float* __restrict__ curr_ = A.curr;
+    const GAS float* __restrict__ curr = gas(A.curr);
+    const GAS float* __restrict__ bb = gas(A.b);
+    const GAS float* __restrict__ K = gas(A.K);
     const int p = g.at(ix, iy, iz);
-    float lap = (1.0f / (g.dx * g.dx)) * d2_point(curr, bb, p, ix, g.nx, g.sx);
-    lap = lap + (1.0f / (g.dy * g.dy)) * d2_point(curr, bb, p, iy, g.ny, g.sy);
-    lap = lap + (1.0f / (g.dz * g.dz)) * d2_point(curr, bb, p, iz, g.nz, 1);
-    float pv = A.prev[p];
+    float lap = (1.0f / (g.dx * g.dx)) * d2_point(curr_, A.b, p, ix, g.nx, g.sx);
+    lap = lap + (1.0f / (g.dy * g.dy)) * d2_point(curr_, A.b, p, iy, g.ny, g.sy);
+    lap = lap + (1.0f / (g.dz * g.dz)) * d2_point(curr_, A.b, p, iz, g.nz, 1);
+    float pv = gld(A.prev, p);
     if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
     float sat = 0.f;
 #pragma unroll
@@ -172,7 +173,7 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b,
             sat += 1.0f / (c_hnorm[k] * d2) * c_sat_s[k] * v;
             if (k == 0) sat -= v / (A.alpha * SAT_H0 * d2);
         } else if (k == 0) {
-            const float sd = sat_s(curr, p, inward);
+            const float sd = sat_s(curr_, p, inward);
             const float imp = (bc == 2) ? 0.5f / sqrtf(K[p] / bb[p]) : 0.f;
             const float adh = 1.0f / (SAT_H0 * h);
             sat -= adh * (bb[p] * sd / h - imp * pv / A.dt);
@@ -181,7 +182,7 @@ __device__ __forceinline__ void acoustic_boundary_block(const AcuArgs& A, int b,
     lap += sat;
     float nv = A.dt2 * lap * K[p] + 2.f * curr[p] - pv;
     nv = nv / acu_scale(g, A.bc, K[p], bb[p], ix, iy, iz, A.dt);
-    A.next[p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+    gst(A.next, p, taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz));
 }
 
 // Tiling of the slab kernel of the elastic / VTI shell (elastic.hpp): the y-marching
@@ -301,28 +302,31 @@ __global__ void acoustic_inject(float* __restrict__ out, AcuArgs A, InjTab I, co
 // component comp = blockIdx.y reads field u + comp * g.n.
 __device__ __forceinline__ void extract_one(const float* __restrict__ u3, const Geom& g, const InjTab& I,
                                             float* __restrict__ rcv, int nt, int it, int r, int comp) {
-    const float* __restrict__ u = u3 + comp * g.n;
-    const float* xw = I.xw + r * 2 * I.nw + I.nw;
-    const float* yw = I.yw + r * 2 * I.nw + I.nw;
-    const float* zw = I.zw + r * 2 * I.nw + I.nw;
+    const GAS float* __restrict__ u = gas(u3 + comp * g.n);
+    const GAS float* xw = gas(I.xw + r * 2 * I.nw + I.nw);
+    const GAS float* yw = gas(I.yw + r * 2 * I.nw + I.nw);
+    const GAS float* zw = gas(I.zw + r * 2 * I.nw + I.nw);
     float acc = 0.f;
     if (I.kind == 2) {  // dipole_m3::extract (injector.cpp:461-492): u(end 2) - u(end 1)
-        const int* xi = I.xi + 6 * r;
-        const int* yi = I.yi + 6 * r;
-        const int* zi = I.zi + 6 * r;
+        const GAS int* xi = gas(I.xi + 6 * r);
+        const GAS int* yi = gas(I.yi + 6 * r);
+        const GAS int* zi = gas(I.zi + 6 * r);
         for (int a = 0; a < 3; a++)
             for (int bb = 0; bb < 3; bb++)
                 for (int c = 0; c < 3; c++)
                     acc += xw[3 + bb] * yw[3 + a] * zw[3 + c] * u[g.at(xi[3 + bb], yi[3 + a], zi[3 + c])] -
                            xw[bb] * yw[a] * zw[c] * u[g.at(xi[bb], yi[a], zi[c])];
-        rcv[(long long)(comp * I.npts + r) * nt + it] = acc;
+        gas(rcv)[(long long)(comp * I.npts + r) * nt + it] = acc;
         return;
     }
+    const GAS int* xi = gas(I.xi + 3 * r);
+    const GAS int* yi = gas(I.yi + 3 * r);
+    const GAS int* zi = gas(I.zi + 3 * r);
     for (int a = 0; a < 3; a++)
         for (int bb = 0; bb < 3; bb++)
             for (int c = 0; c < 3; c++)
-                acc += yw[a] * xw[bb] * zw[c] * u[g.at(I.xi[3 * r + bb], I.yi[3 * r + a], I.zi[3 * r + c])];
-    rcv[(long long)(comp * I.npts + r) * nt + it] = acc;
+                acc += yw[a] * xw[bb] * zw[c] * u[g.at(xi[bb], yi[a], zi[c])];
+    gas(rcv)[(long long)(comp * I.npts + r) * nt + it] = acc;
 }
 
 // Imaging condition of nl_we_op_a::compute_gradients (we_op.cpp:1555-1605), FP64

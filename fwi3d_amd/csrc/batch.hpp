@@ -26,7 +26,7 @@
 
 struct fwi3d_batch {
     enum Cls { C_AINT, C_ASHELL, C_AINJ, C_EFLUX, C_EINT, C_ESHELL, C_EZS, C_EINJ, C_VFLUX, C_VINT, C_VSHELL, C_VZS,
-               C_VINJ, C_EXT, NCLS };
+               C_VINJ, C_EXT, C_EFUSED, C_VFUSED, C_EFEDGE, C_VFEDGE, C_EYS, C_VYS, NCLS };
     struct Shot {
         int id = -1;
         fwi3d::BShot d{};
@@ -146,7 +146,7 @@ void batch_retile(fwi3d_batch* b, int yc) {
         BShot& d = sh.d;
         if (d.phys == FWI3D_ACOUSTIC) {
             sh.nblk[B::C_AINT] = tile_grid_yc(d.a, d.g, b->TX, true, yc);
-        } else {
+        } else if (!d.fused) {  // (the fused kernel keeps its own y chunk)
             const bool e = d.phys == FWI3D_ELASTIC;
             sh.nblk[e ? B::C_EFLUX : B::C_VFLUX] = tile_grid_yc(d.ef, d.g, b->TX, false, yc);
             sh.nblk[e ? B::C_EINT : B::C_VINT] = tile_grid_yc(d.ei, d.g, b->TX, true, yc);
@@ -230,6 +230,10 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
         go(B::C_EINT, elastic_interior_batch<4, false>, 0, dim3(64, 4));
         go(B::C_VINT, elastic_interior_batch<4, true>, 0, dim3(64, 4));
     }
+    go(B::C_EFUSED, elastic_fused_batch<false, false>, 0, dim3(FTHREADS));
+    go(B::C_VFUSED, elastic_fused_batch<true, false>, 0, dim3(FTHREADS));
+    go(B::C_EFEDGE, elastic_fused_batch<false, true>, 0, dim3(FTHREADS));
+    go(B::C_VFEDGE, elastic_fused_batch<true, true>, 0, dim3(FTHREADS));
     go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
     go(B::C_ESHELL, elastic_boundary_batch<false>, 0, dim3(256));
     go(B::C_VSHELL, elastic_boundary_batch<true>, 0, dim3(256));
@@ -237,6 +241,8 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
     // shell classes stay empty)
     go(B::C_EZS, elastic_edges_slabs_batch<false>, 0, dim3(256));
     go(B::C_VZS, elastic_edges_slabs_batch<true>, 0, dim3(256));
+    go(B::C_EYS, elastic_yslabs_batch<false>, 0, dim3(256));
+    go(B::C_VYS, elastic_yslabs_batch<true>, 0, dim3(256));
     go(B::C_AINJ, acoustic_inject_batch, 0, dim3(256));
     go(B::C_EINJ, elastic_inject_batch<false>, 0, dim3(256));
     go(B::C_VINJ, elastic_inject_batch<true>, 0, dim3(256));
@@ -447,6 +453,17 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             d.nshell_blk = (d.nshell_pts + 255) / 256;
             s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 0;  // the edges run in the slab class
             s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g) + 3 * d.nshell_blk;
+            if (c->use_fused) {
+                // the same plan as the single-shot path, so a shot computes the same bits
+                // alone and in a batch
+                const FusedPlan P = fused_plan(d, g, TX, yc, c->fused_chunk);
+                s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = P.nflux;
+                s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = 0;
+                s.nblk[grp == 1 ? B::C_EFUSED : B::C_VFUSED] = P.nfused;
+                s.nblk[grp == 1 ? B::C_EFEDGE : B::C_VFEDGE] = P.nfedge;
+                s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = 0;
+                s.nblk[grp == 1 ? B::C_EYS : B::C_VYS] = P.nslab;
+            }
             s.nblk[grp == 1 ? B::C_EINJ : B::C_VINJ] = 3 * d.ninj_blk;
             A.prev = F; A.curr = F + 3 * n; A.next = F + 6 * n;
             if (d.I.ntgt > 0) {

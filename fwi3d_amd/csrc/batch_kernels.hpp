@@ -8,6 +8,7 @@
 #pragma once
 #include "acoustic.hpp"
 #include "elastic.hpp"
+#include "fused.hpp"
 
 namespace fwi3d {
 
@@ -23,8 +24,13 @@ struct BShot {
     const float* src;
     float* rcv;
     int nshell_blk;  // shell workgroups of one component (256 points each)
-    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the edges and corners)
+    int nshell_pts;  // shell points computed by the shell kernel (elastic / VTI: the edges and
+                     // corners; with the fused kernel the edges of the y slabs only)
     ZSlabTiles zs;   // elastic / VTI: tiling of the slab kernel (z slabs, then the side slabs)
+    int fused;       // elastic / VTI: the fused kernel updates the planes [6, ny-6), the slab
+                     // launch only the y slabs (side tiles and edges), the flux pass ef
+                     // covers the 8 planes at each y face
+    FusedTiles ft;   // tiling of the fused kernel
     int ninj_blk;    // injection workgroups of one component (256 targets each)
     int next_blk;    // extraction workgroups of one component (128 receivers)
 };
@@ -172,6 +178,46 @@ __global__ __launch_bounds__(64 * TX) __attribute__((amdgpu_waves_per_eu(4))) vo
     elastic_interior_tile<TX, VTI>(A, [&] { return l; });
 }
 
+// Fused step over the planes [6, ny-6) (fused.hpp), 512 threads per workgroup: the
+// middle tiles (EDGE false) or the edge tiles of each y chunk (EDGE true).
+template <bool VTI, bool EDGE>
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(VTI ? (EDGE ? 2 : 3) : (EDGE ? 2 : 4)))) void elastic_fused_batch(
+    const BShot* __restrict__ T, BMap m, StepIdx it) {
+    __shared__ FusedSmem<VTI> smem;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = m.shot_of(L);
+    const int k = it.value() - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    const FusedTiles& FT = T[s].ft;
+    const int l = L - m.first(s);
+    if (EDGE) {
+        const int per = fused_edge_tiles(FT.ntz, FT.ntx);
+        int tz, tx;
+        fused_edge_tile(l % per, FT.ntz, FT.ntx, tz, tx);
+        elastic_fused_tile<VTI, true, true>(A, FT, tz, tx, l / per, smem);
+    } else {
+        const int mz = FT.ntz - 2, per = mz * (FT.ntx - 2);
+        const int r = l % per;
+        elastic_fused_tile<VTI, false, false>(A, FT, 1 + r % mz, 1 + r / mz, l / per, smem);
+    }
+}
+
+// Slab workgroup l of a shot whose planes [6, ny-6) the fused kernel updates: the y-slab
+// side tiles (elastic_side_tile<1>), then the edges of the y slabs (3 components).
+template <bool VTI>
+__device__ __forceinline__ void elastic_yslab_block(const EArgs& A, const BShot& S, int l, float* smem) {
+    const ZSlabTiles Z = S.zs;
+    const int nside = 2 * Z.sntz * Z.snc1;
+    if (l < nside) {
+        elastic_side_tile<1, VTI>(A, Z, l, smem);
+        return;
+    }
+    const int e = l - nside, nb = S.nshell_blk;
+    elastic_boundary_block<VTI>(A, e % nb, e / nb, S.nshell_pts);
+}
+
 // Edges and corners of the shell, point-wise. Workgroups are not XCD-remapped: dispatch
 // order gives every XCD the same mix of the rows along x and the z columns.
 // Occupancy: 5 waves per SIMD measured faster than 6 and 7.
@@ -288,6 +334,37 @@ __global__ __launch_bounds__(256) void elastic_edges_slabs_batch(const BShot* __
     }
     const int e = l - nslab, nb = T[s].nshell_blk;
     elastic_boundary_block<VTI>(A, e % nb, e / nb, T[s].nshell_pts);
+}
+
+// Single-shot path with the fused kernel: the y-slab side tiles and edges (nslab
+// workgroups), then the receiver extraction of the same step. A kernel of its own, so
+// its code holds only these tiles (the instruction cache is the limit of the slab code).
+template <bool VTI>
+__global__ __launch_bounds__(256) void elastic_yslabs_ext(const BShot* __restrict__ T, StepIdx it, int nslab) {
+    __shared__ float smem[slab_lds_floats<VTI>()];
+    const int k = it.value() - T[0].t0;
+    if (k < 0 || k > T[0].nt - 2) return;
+    const int L = blockIdx.x;
+    if (L >= nslab) {
+        extract_block256(T[0], L - nslab, k);
+        return;
+    }
+    EArgs A = T[0].ef;
+    bshot_rotate(A, T[0], k);
+    elastic_yslab_block<VTI>(A, T[0], L, smem);
+}
+
+// Batch form (receivers are a class of their own in a batch).
+template <bool VTI>
+__global__ __launch_bounds__(256) void elastic_yslabs_batch(const BShot* __restrict__ T, BMap m, StepIdx it) {
+    __shared__ float smem[slab_lds_floats<VTI>()];
+    const int L = blockIdx.x;
+    const int s = m.shot_of(L);
+    const int k = it.value() - T[s].t0;
+    if (k < 0 || k > T[s].nt - 2) return;
+    EArgs A = T[s].ef;
+    bshot_rotate(A, T[s], k);
+    elastic_yslab_block<VTI>(A, T[s], L - m.first(s), smem);
 }
 
 // ---- source injection into `next` after the step (mode 0), receivers before it.

@@ -27,6 +27,25 @@ struct Geom {
     __host__ __device__ int at(int ix, int iy, int iz) const { return iy * sy + ix * sx + iz; }
 };
 
+// Global-memory pointers in address space 1. The step kernels read their field pointers
+// from the per-shot table in device memory, so the compiler cannot infer the address
+// space of those pointers and would emit generic (FLAT) accesses, which also count
+// against the LDS wait counter: every LDS barrier would wait for all prefetched global
+// loads, and each access would need 64-bit address arithmetic. Casting to GAS pointers
+// and indexing with 32-bit unsigned element offsets gives global loads with a scalar
+// base and a 32-bit vector offset.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GAS T* gas(const T* p) { return (const GAS T*)p; }
+template <class T>
+__device__ __forceinline__ GAS T* gas(T* p) { return (GAS T*)p; }
+// (the byte offset is formed in 32 bits, so the access is base + zext(offset): a scalar
+// base register pair and one 32-bit vector offset)
+__device__ __forceinline__ float gld(const float* base, unsigned i) {
+    return *(const GAS float*)((const GAS char*)gas(base) + (i << 2));
+}
+__device__ __forceinline__ void gst(float* base, unsigned i, float v) { *(GAS float*)((GAS char*)gas(base) + (i << 2)) = v; }
+
 // S and H weights (the closure rows are compile-time constants, below).
 __constant__ float c_sat_s[4];
 __constant__ float c_hnorm[4];
@@ -115,8 +134,10 @@ __device__ __forceinline__ float d1_rows(int r, const U& u) {
 // i >= n-6) along z (s == 1) the lanes of a wave sit on different rows, so the eight
 // u and b values are loaded once for all lanes and each row's terms then run from
 // registers; along x / y (one row per wave) the row loads only its own operands.
-__device__ __forceinline__ float d2_point(const float* __restrict__ u, const float* __restrict__ b, int p, int i, int n,
+__device__ __forceinline__ float d2_point(const float* __restrict__ u_, const float* __restrict__ b_, int p, int i, int n,
                                           int s) {
+    const GAS float* __restrict__ u = gas(u_);
+    const GAS float* __restrict__ b = gas(b_);
     if (i >= 6 && i < n - 6) {
         return d2_interior(u[p - 2 * s], u[p - s], u[p], u[p + s], u[p + 2 * s], b[p - 2 * s], b[p - s], b[p],
                            b[p + s], b[p + 2 * s]);
@@ -138,7 +159,8 @@ __device__ __forceinline__ float d2_point(const float* __restrict__ u, const flo
 }
 // D1 (times h) on a boundary row (i < 4 or i >= n-4), reading global memory (row
 // forms as for d2_point).
-__device__ __forceinline__ float d1_closure(const float* __restrict__ u, int p, int i, int n, int s) {
+__device__ __forceinline__ float d1_closure(const float* __restrict__ u_, int p, int i, int n, int s) {
+    const GAS float* __restrict__ u = gas(u_);
     const bool lo = i < 4;
     const int r = lo ? i : n - 1 - i;
     const int base = lo ? p - i * s : p + (n - 1 - i) * s;
@@ -166,13 +188,15 @@ __device__ __forceinline__ float d1_closure_at(bool lo, int r, const A_& a) {
     return lo ? v : -v;
 }
 // D1 (times h) at axis index i reading global memory.
-__device__ __forceinline__ float d1_point(const float* __restrict__ u, int p, int i, int n, int s) {
+__device__ __forceinline__ float d1_point(const float* __restrict__ u_, int p, int i, int n, int s) {
+    const GAS float* __restrict__ u = gas(u_);
     if (i >= 4 && i < n - 4) return D1C1 * (u[p + s] - u[p - s]) + D1C2 * (u[p + 2 * s] - u[p - 2 * s]);
-    return d1_closure(u, p, i, n, s);
+    return d1_closure(u_, p, i, n, s);
 }
 
 // Outward boundary derivative S (times h) at a face point p, `inward` = +/- stride.
-__device__ __forceinline__ float sat_s(const float* __restrict__ u, int p, int inward) {
+__device__ __forceinline__ float sat_s(const float* __restrict__ u_, int p, int inward) {
+    const GAS float* __restrict__ u = gas(u_);
     return c_sat_s[0] * u[p] + c_sat_s[1] * u[p + inward] + c_sat_s[2] * u[p + 2 * inward] + c_sat_s[3] * u[p + 3 * inward];
 }
 
@@ -225,7 +249,7 @@ struct Bound {
 // taper: ((v * fz) * fx) * fy, the order of the reference's six taper calls.
 __device__ __forceinline__ float taper_apply(float v, const float* __restrict__ tz2, const float* __restrict__ tx2,
                                              const float* __restrict__ ty2, int ix, int iy, int iz) {
-    return ((v * tz2[iz]) * tx2[ix]) * ty2[iy];
+    return ((v * gld(tz2, iz)) * gld(tx2, ix)) * gld(ty2, iy);
 }
 
 // Workgroup -> tile remap so that the workgroups one XCD receives (hardware assigns

@@ -47,8 +47,10 @@ struct EArgs {
     float dt, dt2;
     int prev_tapered;
     // y-march tiling: tiles of 64 (z) x TX (x) x ychunk (y) from the corner (org, org, org);
-    // the y march of the last chunk stops at yend
-    int ychunk, ntz, ntx, nyc, org, yend;
+    // the y march of the last chunk stops at yend. ysplit (flux pass only): two chunks,
+    // the 8 planes at each y face (the planes the slab and edge kernels read fluxes from
+    // when the fused kernel updates the planes between them)
+    int ychunk, ntz, ntx, nyc, org, yend, ysplit;
 };
 
 // ------------------------------------------------------------------------------------
@@ -59,12 +61,13 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
     constexpr int TZ = 64;
     __shared__ float su[3][TX + 4][TZ + 4];
     const float* __restrict__ curr = A.curr;
-    float* __restrict__ aux = A.aux;
-    const float* __restrict__ lamf = A.lam;
-    const float* __restrict__ muf = A.mu;
-    const float* __restrict__ c13f = A.c13;
-    const float* __restrict__ epsf = A.eps;
-    const float* __restrict__ c66f = A.c66;
+    const GAS float* __restrict__ gcur = gas(A.curr);
+    GAS float* __restrict__ aux = gas(A.aux);
+    const GAS float* __restrict__ lamf = gas(A.lam);
+    const GAS float* __restrict__ muf = gas(A.mu);
+    const GAS float* __restrict__ c13f = gas(A.c13);
+    const GAS float* __restrict__ epsf = gas(A.eps);
+    const GAS float* __restrict__ c66f = gas(A.c66);
     const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
     const float dx = A.g.dx, dy = A.g.dy, dz = A.g.dz;
     // y chunks fastest in the workgroup order: the chunks above and below, which re-read
@@ -75,10 +78,11 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
     const int lz = threadIdx.x, lx = threadIdx.y;
     const int z0 = A.org + tz * TZ, x0 = A.org + tx * TX, iz = z0 + lz, ix = x0 + lx;
     const bool valid = (iz < nz) && (ix < nx);
-    const int y0 = A.org + cy * A.ychunk, y1 = min(A.yend, y0 + A.ychunk);
+    const int y0 = A.ysplit ? (cy == 0 ? 0 : ny - 8) : A.org + cy * A.ychunk;
+    const int y1 = A.ysplit ? y0 + 8 : min(A.yend, y0 + A.ychunk);
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
-    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    auto ldy = [&](const float* __restrict__ f, int y) { return gas(f)[max(0, min(ny - 1, y)) * sy + col]; };
     HaloMap<TX> H;
     H.init(lx, lz, x0, z0, nx, nz, sx);
     const bool inx = ix >= 4 && ix < nx - 4, inz = iz >= 4 && iz < nz - 4;
@@ -102,8 +106,8 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
         const int pl = min(y, ny - 1) * sy;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            if (H.has_x) hx[c] = curr[c * n3 + pl + H.xsrc];
-            if (H.has_z) hz[c] = curr[c * n3 + pl + H.zsrc];
+            if (H.has_x) hx[c] = gcur[c * n3 + pl + H.xsrc];
+            if (H.has_z) hz[c] = gcur[c * n3 + pl + H.zsrc];
         }
     };
     prefetch(y0);
@@ -165,7 +169,7 @@ __device__ __forceinline__ void elastic_flux_tile(const EArgs A, const LFn& tile
                 nT = 3;
             }
             const float gxy = VTI ? mq[NMQ - 1] : mu;
-            float* __restrict__ F = aux + nT * n3;
+            GAS float* __restrict__ F = aux + nT * n3;
             F[FX_YX * n3 + p] = dxu[1] * gxy;  // G * D_x u_y  (outer D_y, component x)
             F[FX_XY * n3 + p] = dyu[0] * gxy;  // G * D_y u_x  (outer D_x, component y)
             F[FX_ZY * n3 + p] = dyu[2] * mu;   // mu * D_y u_z (outer D_z, component y)
@@ -221,9 +225,9 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
     const int nx = A.g.nx, ny = A.g.ny, nz = A.g.nz, sx = A.g.sx, sy = A.g.sy, n3 = A.g.n;
     const float dx = A.g.dx, dy = A.g.dy, dz = A.g.dz;
     const float* __restrict__ curr = A.curr;
-    const float* __restrict__ prev = A.prev;
-    float* __restrict__ next = A.next;
-    const float* __restrict__ rhof = A.rho;
+    const GAS float* __restrict__ prev = gas(A.prev);
+    GAS float* __restrict__ next = gas(A.next);
+    const GAS float* __restrict__ rhof = gas(A.rho);
     const float* __restrict__ tz2 = A.tz2;
     const float* __restrict__ tx2 = A.tx2;
     const float* __restrict__ ty2 = A.ty2;
@@ -239,7 +243,7 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
     const int cz = min(iz, nz - 1), cx = min(ix, nx - 1);
     const int col = cx * sx + cz;
     const bool comp_xz = ix >= 6 && ix < nx - 6 && iz >= 6 && iz < nz - 6;
-    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    auto ldy = [&](const float* __restrict__ f, int y) { return gas(f)[max(0, min(ny - 1, y)) * sy + col]; };
     HaloMap<TX> H;
     H.init(lx, lz, x0, z0, nx, nz, sx);
 
@@ -293,14 +297,14 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
         const int pl = min(y, ny - 1) * sy;
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
-            const float* __restrict__ f = psrc[k];
+            const GAS float* __restrict__ f = gas(psrc[k]);
             if (k >= 3 + NB) pc[k] = f[pl + col];
             if (along_x(k) && H.has_x) hx[k] = f[pl + H.xsrc];
             if (along_z(k) && H.has_z) hz[k] = f[pl + H.zsrc];
         }
         if (has_c) {
-            hcx = curr[pl + csrc];
-            hcz = curr[2 * n3 + pl + csrc];
+            hcx = gas(curr)[pl + csrc];
+            hcz = gas(curr)[2 * n3 + pl + csrc];
         }
     };
     prefetch(y0);
@@ -401,7 +405,9 @@ __device__ __forceinline__ void elastic_interior_tile(const EArgs& __restrict__ 
 // Pass 2, boundary shell: one thread per point, every operand from global memory.
 template <bool VTI>
 struct ECoef {
-    const float *lam, *mu, *rho, *c13, *eps, *c66;
+    const GAS float *lam, *mu, *rho, *c13, *eps, *c66;
+    __device__ ECoef(const float* l, const float* m, const float* r, const float* c, const float* e, const float* s)
+        : lam(gas(l)), mu(gas(m)), rho(gas(r)), c13(gas(c)), eps(gas(e)), c66(gas(s)) {}
     // normal-traction coefficient of D_a u_a on a face with normal axis n (0 x, 1 y, 2 z)
     __device__ float cnorm(int n, int a, int p) const {
         if (!VTI) return lam[p];
@@ -494,7 +500,7 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
             acc = acc + D(F + FX_XZ * n3, 0) / g.dx;
             acc = acc + D(F + FX_YZ * n3, 1) / g.dy;
         }
-        float pv = A.prev[c * n3 + p];
+        float pv = gld(A.prev, c * n3 + p);
         if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
         if (face) {
             // traction SATs of the faces through p (reference face order)
@@ -527,9 +533,9 @@ __device__ __forceinline__ void elastic_boundary_point(const EArgs& A, int ix, i
                 acc -= adh * term;
             }
         }
-        float nv = A.dt2 * acc / rho + 2.f * u[p] - pv;
+        float nv = A.dt2 * acc / rho + 2.f * gld(u, p) - pv;
         if (face) nv = nv / e_scale<VTI>(g, A.bc, M, p, ix, iy, iz, c, A.dt);
-        A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+        gst(A.next, c * n3 + p, taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz));
     }
 }
 
@@ -591,7 +597,7 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
     const bool hi = slab != 0;
     const int r = hi ? 7 - lz : lz;  // row counted from the face
     const bool face = r == 0;
-    auto ldy = [&](const float* __restrict__ f, int y) { return f[max(0, min(ny - 1, y)) * sy + col]; };
+    auto ldy = [&](const float* __restrict__ f, int y) { return gas(f)[max(0, min(ny - 1, y)) * sy + col]; };
     // x-halo value of this thread: threads 0..31 load rows x0-2, x0-1, x0+TXZ, x0+TXZ+1
     const int h = threadIdx.x;
     const bool has_h = h < 32;
@@ -634,8 +640,8 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
         const int pl = min(y, ny - 1) * sy;
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
-            if (k >= 3 + NB) pc[k] = psrc[k][pl + col];
-            hv[k] = psrc[k][pl + hoff];
+            if (k >= 3 + NB) pc[k] = gas(psrc[k])[pl + col];
+            hv[k] = gas(psrc[k])[pl + hoff];
         }
     };
     prefetch(y0);
@@ -676,7 +682,7 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
             auto sats = [&](int k) {  // outward S (times h) at the face point
                 return c_sat_s[0] * FZ(k, 0) + c_sat_s[1] * FZ(k, 1) + c_sat_s[2] * FZ(k, 2) + c_sat_s[3] * FZ(k, 3);
             };
-            const float rho = A.rho[p];
+            const float rho = gld(A.rho, p);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2(c, d2_bfield<VTI>(c, 0), 0);
@@ -695,7 +701,7 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
                     acc = acc + d1x(PF + 3) / g.dx;     // D_x F_xz
                     acc = acc + d1y(5 + NB) / g.dy;     // D_y F_yz
                 }
-                float pv = prev[c * n3 + p];
+                float pv = gld(prev, c * n3 + p);
                 if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
                 const int bc = hi ? A.bc[1] : A.bc[0];  // constant indices (no stack copy of A)
                 if (face && (bc == 1 || bc == 2)) {
@@ -723,7 +729,7 @@ __device__ __forceinline__ void elastic_zslab_tile(const EArgs& __restrict__ A, 
                     const float v = c == 2 ? sqrtf((M.lam[p] + 2.f * M.mu[p]) / M.rho[p]) : sqrtf(M.mu[p] / M.rho[p]);
                     nv = nv / (1.f + (0.f + v * A.dt / (2.f * g.dz * SAT_H0)));
                 }
-                A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+                gst(A.next, c * n3 + p, taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz));
             }
         }
 #pragma unroll
@@ -774,7 +780,7 @@ __device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, c
     const bool face = r == 0;
     const int m0 = 6 + ch * Z.schunk, m1 = min(nM - 6, m0 + Z.schunk);
     const int col = ia * sA + min(iz, nz - 1);
-    auto ldm = [&](const float* __restrict__ f, int m) { return f[max(0, min(nM - 1, m)) * sM + col]; };
+    auto ldm = [&](const float* __restrict__ f, int m) { return gas(f)[max(0, min(nM - 1, m)) * sM + col]; };
     // z-halo value of this thread: threads 0..31 load rows 0..7 at z0-2, z0-1, z0+32, z0+33
     const int h = threadIdx.x;
     const bool has_h = h < 32;
@@ -831,8 +837,8 @@ __device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, c
         const int pl = min(m, nM - 1) * sM;
 #pragma unroll
         for (int k = 0; k < NPL; k++) {
-            if (k >= 3 + NB) pc[k] = psrc[k][pl + col];
-            hv[k] = psrc[k][pl + hoff];
+            if (k >= 3 + NB) pc[k] = gas(psrc[k])[pl + col];
+            hv[k] = gas(psrc[k])[pl + hoff];
         }
     };
     prefetch(m0);
@@ -876,7 +882,7 @@ __device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, c
             auto sats = [&](int k) {  // outward S (times h) at the face point
                 return c_sat_s[0] * FN(k, 0) + c_sat_s[1] * FN(k, 1) + c_sat_s[2] * FN(k, 2) + c_sat_s[3] * FN(k, 3);
             };
-            const float rho = A.rho[p];
+            const float rho = gld(A.rho, p);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float acc = (d2_factor<VTI>(c, 0) / (g.dx * g.dx)) * d2(c, d2_bfield<VTI>(c, 0), 0);
@@ -911,7 +917,7 @@ __device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, c
                         acc = acc + dA(PF + 3) / g.dy;      // D_y F_yz
                     }
                 }
-                float pv = prev[c * n3 + p];
+                float pv = gld(prev, c * n3 + p);
                 if (A.prev_tapered) pv = taper_apply(pv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
                 if (face && (bcf == 1 || bcf == 2)) {
                     // traction SAT of this face (elastic_boundary_point, normal axis AX;
@@ -941,7 +947,7 @@ __device__ __forceinline__ void elastic_side_tile(const EArgs& __restrict__ A, c
                     const float v = c == AX ? sqrtf(F * (M.lam[p] + 2.f * M.mu[p]) / M.rho[p]) : sqrtf(M.mu[p] / M.rho[p]);
                     nv = nv / (1.f + (0.f + v * A.dt / (2.f * hA * SAT_H0)));
                 }
-                A.next[c * n3 + p] = taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz);
+                gst(A.next, c * n3 + p, taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, iy, iz));
             }
         }
 #pragma unroll

@@ -349,6 +349,56 @@ int zslab_tiles(ZSlabTiles& Z, const Geom& g) {
 // and corners (edge_point's enumeration).
 int shell_edge_count(const Geom& g) { return edge_count(6, g.nx, g.ny, g.nz); }
 
+// Elastic / VTI step with the fused kernel (fused.hpp): its tiling over the planes
+// [6, ny-6); the flux pass (d.ef) over the 8 planes at each y face, or the whole grid
+// when ny < 16; the slab launch over the y slabs: side tiles, then the y-slab points
+// within 6 cells of an x or z face (the first part of edge_point's enumeration).
+struct FusedPlan {
+    int nfused, nfedge, nflux, nslab;  // fused middle / edge workgroups, flux pass, slab launch
+};
+FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk) {
+    FusedPlan P{};
+    d.fused = 1;
+    FusedTiles& F = d.ft;
+    F.ntz = fused_tile_count(g.nz, FZ);
+    F.ntx = fused_tile_count(g.nx, FW);
+    F.ya = 6;
+    F.yb = g.ny - 6;
+    const int nyo = std::max(0, F.yb - F.ya);
+    const int tiles = F.ntz * F.ntx;
+    int yc = fchunk;
+    if (yc <= 0) {
+        // at most 512 workgroups of middle tiles (two per CU, one round), chunks of at
+        // least 8 planes (each chunk spends 4 iterations filling its queues)
+        const int mid = fused_middle_tiles(F.ntz, F.ntx);
+        const int want = std::max(1, 512 / std::max(1, mid > 0 ? mid : tiles));
+        yc = std::max(8, (nyo + want - 1) / want);
+    }
+    F.ychunk = std::max(1, std::min(yc, std::max(nyo, 1)));
+    F.nyc = nyo > 0 ? (nyo + F.ychunk - 1) / F.ychunk : 0;
+    P.nfused = fused_middle_tiles(F.ntz, F.ntx) * F.nyc;
+    P.nfedge = fused_edge_tiles(F.ntz, F.ntx) * F.nyc;
+    EArgs& A = d.ef;
+    if (g.ny >= 16) {
+        A.org = 0;
+        A.yend = g.ny;
+        A.ntz = (g.nz + 63) / 64;
+        A.ntx = (g.nx + TX - 1) / TX;
+        A.ychunk = 8;
+        A.nyc = 2;
+        A.ysplit = 1;
+        P.nflux = A.ntz * A.ntx * A.nyc;
+    } else {
+        A.ysplit = 0;
+        P.nflux = tile_grid_yc(A, g, TX, false, ychunk);
+    }
+    zslab_tiles(d.zs, g);
+    d.nshell_pts = 2 * 6 * (2 * 6 * g.nz + (g.nx - 2 * 6) * 2 * 6);
+    d.nshell_blk = (d.nshell_pts + 255) / 256;
+    P.nslab = 2 * d.zs.sntz * d.zs.snc1 + 3 * d.nshell_blk;
+    return P;
+}
+
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
 // grid on the context, else the context's settings (fwi3d_tune).
 void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk) {
@@ -597,6 +647,27 @@ __global__ void vti_c11b_field(float* __restrict__ c11b, const float* __restrict
 // The one-shot table T carries the flux pass's tiling (whole grid, nflux workgroups)
 // and the interior's (nint workgroups, 0 when the interior is empty).
 // next_blk: receiver-extraction workgroups launched with the shell (0: none this step).
+// One elastic/VTI time step with the fused kernel: fused planes, flux pass at the y
+// faces, y slabs with the receivers of the step.
+template <int TX, bool VTI>
+void step_elastic_fused(fwi3d_ctx* c, const BShot* T, StepIdx it, const FusedPlan& P, int next_blk) {
+    if (P.nfused + P.nfedge > 0)
+        timed(c, 0, [&] {
+            if (P.nfused > 0)
+                hipLaunchKernelGGL((elastic_fused_batch<VTI, false>), dim3(P.nfused), dim3(FTHREADS), 0, c->stream, T,
+                                   BMap{}, it);
+            if (P.nfedge > 0)
+                hipLaunchKernelGGL((elastic_fused_batch<VTI, true>), dim3(P.nfedge), dim3(FTHREADS), 0, c->stream, T,
+                                   BMap{}, it);
+        });
+    timed(c, 1, [&] {
+        hipLaunchKernelGGL((elastic_flux_batch<TX, VTI>), dim3(P.nflux), dim3(64, TX), 0, c->stream, T, BMap{}, it);
+    });
+    timed(c, 3, [&] {
+        hipLaunchKernelGGL(elastic_yslabs_ext<VTI>, dim3(P.nslab + next_blk), dim3(256), 0, c->stream, T, it, P.nslab);
+    });
+}
+
 template <int TX, bool VTI>
 void step_elastic(fwi3d_ctx* c, const BShot* T, StepIdx it, int nflux, int nint, int nshell_blk, int nzslab,
                   int next_blk) {
@@ -688,6 +759,9 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     T1h.nshell_pts = shell_edge_count(g);
     T1h.nshell_blk = (T1h.nshell_pts + 255) / 256;
     const int nzslab = zslab_tiles(T1h.zs, g);
+    const bool fused = c->use_fused;
+    FusedPlan FP{};
+    if (fused) FP = fused_plan(T1h, g, TX, ychunk, c->fused_chunk);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     T1h.E = E; T1h.rcv = d_rcv; T1h.next_blk = (E.npts + 127) / 128;
     const BShot* T1 = upload_shot1(c, T1h);
@@ -705,6 +779,16 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int ext_blk = (!adj && E.npts > 0) ? extract_blocks256(E.npts, 3) : 0;
     auto step = [&](StepIdx si) {
         timed(c, 4, [&] {
+            if (fused) {
+                if (vti) {
+                    if (TX == 8) step_elastic_fused<8, true>(c, T1, si, FP, ext_blk);
+                    else step_elastic_fused<4, true>(c, T1, si, FP, ext_blk);
+                } else {
+                    if (TX == 8) step_elastic_fused<8, false>(c, T1, si, FP, ext_blk);
+                    else step_elastic_fused<4, false>(c, T1, si, FP, ext_blk);
+                }
+                return;
+            }
             if (vti) {
                 if (TX == 8) step_elastic<8, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
                 else step_elastic<4, true>(c, T1, si, nblk, nint, T1h.nshell_blk, nzslab, ext_blk);
@@ -1000,6 +1084,16 @@ int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
     if (param == FWI3D_TUNE_GRAPHS) {
         if (value != 0 && value != 1) return fail(c, Err{FWI3D_EINVAL, "graphs must be 0 or 1"});
         c->use_graphs = value != 0;
+        return FWI3D_OK;
+    }
+    if (param == FWI3D_TUNE_FUSED) {
+        if (value != 0 && value != 1) return fail(c, Err{FWI3D_EINVAL, "fused must be 0 or 1"});
+        c->use_fused = value != 0;
+        return FWI3D_OK;
+    }
+    if (param == FWI3D_TUNE_FCHUNK) {
+        if (value < 0) return fail(c, Err{FWI3D_EINVAL, "fused y chunk must be >= 0 (0: by grid size)"});
+        c->fused_chunk = value;
         return FWI3D_OK;
     }
     return fail(c, Err{FWI3D_EINVAL, "unknown tuning parameter"});

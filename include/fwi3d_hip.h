@@ -120,9 +120,19 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
  * workgroup (0: default 8), FWI3D_TUNE_TX = x rows per workgroup (4 or 8).
  * FWI3D_TUNE_GRAPHS (default 1): time loops with no host work between steps (forward
  * propagation without snapshots, adjoint propagation without a gradient) replay a
- * captured hipGraph of 16 steps instead of launching each kernel. Results depend on
- * none of these settings. */
-enum fwi3d_tune_param { FWI3D_TUNE_YCHUNK = 0, FWI3D_TUNE_TX = 1, FWI3D_TUNE_GRAPHS = 2 };
+ * captured hipGraph of 16 steps instead of launching each kernel.
+ * FWI3D_TUNE_FUSED (default 0): elastic / VTI steps update the planes 6 cells inside the
+ * y faces with the fused kernel (stresses and fluxes formed on chip; 0: the flux pass,
+ * interior, slab and edge kernels over the whole grid). FWI3D_TUNE_FCHUNK: y planes per
+ * workgroup of the fused kernel (0: chosen from the grid size). Results agree to
+ * rounding between the two elastic forms and depend on none of the other settings. */
+enum fwi3d_tune_param {
+    FWI3D_TUNE_YCHUNK = 0,
+    FWI3D_TUNE_TX = 1,
+    FWI3D_TUNE_GRAPHS = 2,
+    FWI3D_TUNE_FUSED = 3,
+    FWI3D_TUNE_FCHUNK = 4
+};
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
 /* Autotuner (no counterpart in the reference): times `steps` time steps of the given
  * physics and grid on a synthetic model for every tile shape (TX 4 or 8, y chunk 4, 8,
