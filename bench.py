@@ -643,8 +643,9 @@ def main():
                     help="skip the mixed-precision 256^3 gradients, the batched VTI comparison and the reduced "
                          "serving stream that the default run adds after the headline")
     ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
-    ap.add_argument("--no-fused", action="store_true",
-                    help="elastic / VTI steps with the flux, interior, slab and edge kernels instead of the fused kernel")
+    ap.add_argument("--fused", action="store_true",
+                    help="elastic / VTI steps with the fused kernel (fwi3d_tune FWI3D_TUNE_FUSED) instead of the flux, "
+                         "interior, slab and edge kernels")
     ap.add_argument("--workload", default="gradient", choices=("gradient", "acoustic", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); acoustic: 100^3 acoustic forward modelling; "
                          "lbfgs: 256^3 mixed-precision l-BFGS FWI; batch: 32 VTI shots on 3 grids, batched vs one at "
@@ -689,8 +690,8 @@ def main():
     nt = shot.nt
     dev = torch.device("cuda:0")
     eng = F.Engine(0)
-    if args.no_fused:
-        eng.tune(fused=False)
+    if args.fused:
+        eng.tune(fused=True)
     geo = F.Shot(shape, shot.d, (0.0, 0.0, 0.0), nt, shot.dt, shot.src_loc[0], shot.rcv_loc)
     # the reference's param for this shot: raw (vp, vs, rho), explosive moment tensor,
     # propagation dt = data dt (sinc resampling still runs), imaging every 4 steps
