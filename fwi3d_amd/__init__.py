@@ -32,7 +32,7 @@ EXPORTS = (
     "fwi3d_batch_snap_footprint", "fwi3d_batch_add_adjoint", "fwi3d_batch_release", "fwi3d_batch_retired",
     "fwi3d_server_create", "fwi3d_server_submit", "fwi3d_server_poll", "fwi3d_server_wait", "fwi3d_server_error",
     "fwi3d_server_result", "fwi3d_server_stats", "fwi3d_server_destroy", "fwi3d_dipole_tables",
-    "fwi3d_snapshots_read",
+    "fwi3d_snapshots_read", "fwi3d_propagate_device_noalloc",
 )
 
 
@@ -298,6 +298,21 @@ class Engine:
                                           C.c_void_p(grad64_ptr) if grad64_ptr else None, C.byref(par), nx, ny, nz,
                                           C.c_float(d[0]), C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device")
+
+    def propagate_device_noalloc(self, physics, model_ptr, src_ptr, rcv_ptr, inj, ext, par, shape, d, adj=False):
+        """Allocation-free device-pointer propagate (fwi3d_propagate_device_noalloc): the
+        shot's buffers live in the library's static device workspace (128 MiB per GPU);
+        no snapshot store (par.sub must be 0) and no gradient. Calls on one GPU must not
+        overlap."""
+        ph = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        ny, nx, nz = shape
+        I = inj.cstruct()
+        E = ext.cstruct() if ext is not None else InjectorStruct(0, 0, None, None, None, None, None, None)
+        rc = lib().fwi3d_propagate_device_noalloc(self._h, ph, int(bool(adj)), C.c_void_p(model_ptr),
+                                                  C.c_void_p(src_ptr), C.c_void_p(rcv_ptr) if rcv_ptr else None,
+                                                  C.byref(I), C.byref(E), C.byref(par), nx, ny, nz, C.c_float(d[0]),
+                                                  C.c_float(d[1]), C.c_float(d[2]))
+        self._check(rc, "fwi3d_propagate_device_noalloc")
 
     def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None):
         """Tiling of the time-step kernels, graph replay of time loops, and the elastic /

@@ -28,12 +28,18 @@ struct Err {
         if (!(cond)) throw Err{FWI3D_EINVAL, std::string(text)}; \
     } while (0)
 
-// Growable device allocation.
+// Growable device allocation. A borrowed buffer points into memory the library does not
+// own (the static workspace of the allocation-free entry point): it is never freed or
+// grown, and asking it for more than it holds is an error.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     void ensure(size_t nb) {
         if (nb <= bytes) return;
+        if (borrowed)
+            throw Err{FWI3D_ENOMEM, "workspace slice of " + std::to_string(bytes) + " bytes is too small for " +
+                                        std::to_string(nb) + " bytes"};
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
@@ -42,9 +48,10 @@ struct DevBuf {
         bytes = nb;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
     template <class T>
     T* as() const { return (T*)p; }

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -26,6 +27,28 @@ using namespace fwi3d;
 namespace {
 
 bool g_tables_loaded[64] = {false};
+
+// Workspace of the allocation-free entry point (fwi3d_propagate_device_noalloc): a
+// file-scope device array, so the library allocates nothing at call time. Each device
+// that loads the library holds its own copy until the process ends. In host code the
+// array's name is only a stand-in: its device address, per device, comes from
+// hipGetSymbolAddress at context creation (scratch_setup) and is the only handle used.
+constexpr size_t kScratchBytes = size_t(128) << 20;  // 128 MiB
+static __device__ __align__(256) char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {nullptr};
+
+void scratch_setup(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= 64 || g_scratch_ptr[dev]) return;
+    void* ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));  // the current device's copy
+    g_scratch_ptr[dev] = ptr;
+}
+void* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (dev >= 0 && dev < 64) ? g_scratch_ptr[dev] : nullptr;
+}
 
 void load_tables(int dev) {
     if (dev >= 0 && dev < 64 && g_tables_loaded[dev]) return;
@@ -867,6 +890,7 @@ int fwi3d_create(fwi3d_ctx** out, int device) {
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         load_tables(device);
+        scratch_setup(device);
     } catch (const Err& e) {
         int code = e.code;
         delete c;
@@ -1019,6 +1043,78 @@ int fwi3d_propagate_device(fwi3d_ctx* c, int physics, int adj, const float* d_mo
         drain_events(c);
         return FWI3D_OK;
     } catch (const Err& e) {
+        return fail(c, e);
+    }
+}
+
+int fwi3d_propagate_device_noalloc(fwi3d_ctx* c, int physics, int adj, const float* d_model, const float* d_src,
+                                   float* d_rcv, const fwi3d_injector* inj, const fwi3d_injector* ext,
+                                   const fwi3d_param* par, int nx, int ny, int nz, float dx, float dy, float dz) {
+    if (!c) return FWI3D_EINVAL;
+    // the context's device buffers of the single-shot path, swapped for slices of the
+    // static workspace for the duration of the call and restored afterwards
+    DevBuf* bufs[7] = {&c->fields, &c->model, &c->aux, &c->tapers, &c->injtab, &c->exttab, &c->shot1_dev};
+    DevBuf saved[7];
+    bool swapped = false;
+    const bool graphs0 = c->use_graphs, prof0 = c->prof;
+    auto restore = [&] {
+        if (swapped)
+            for (int k = 0; k < 7; k++) *bufs[k] = saved[k];
+        swapped = false;
+        c->use_graphs = graphs0;
+        c->prof = prof0;
+    };
+    try {
+        validate(physics, par, nx, ny, nz, dx, dy, dz);
+        REQUIRE(d_model && d_src, "null model or source");
+        REQUIRE(par->sub == 0, "the allocation-free entry point keeps no snapshot store: sub must be 0 "
+                               "(use fwi3d_propagate_device for imaging runs)");
+        check_injector(inj, nx, ny, nz, "source");
+        if (!adj) {
+            check_injector(ext, nx, ny, nz, "receiver");
+            REQUIRE(d_rcv || ext->npts == 0, "null receiver buffer");
+        }
+        HIP_OK(hipSetDevice(c->device));
+        char* ws = (char*)scratch_ptr(c->device);
+        REQUIRE(ws != nullptr, "no workspace address for this device");
+        Geom g = make_geom(nx, ny, nz, dx, dy, dz);
+        const size_t n = (size_t)g.n;
+        const bool ac = physics == FWI3D_ACOUSTIC;
+        const int ncomp = ac ? 1 : 3, nmf = ac ? 2 : (physics == FWI3D_ELASTIC ? 3 : 7);
+        const int naux = ac ? 0 : (physics == FWI3D_ELASTIC ? 7 : 9);
+        auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+        const size_t sz[7] = {al(3 * ncomp * n * 4), al(nmf * n * 4), al(naux * n * 4),
+                              al((size_t)(g.nz + g.nx + g.ny) * 4), al(inj_bytes(inj)),
+                              adj ? 0 : al(inj_bytes(ext)), al(sizeof(BShot))};
+        size_t total = 0;
+        for (size_t x : sz) total += x;
+        REQUIRE(total <= kScratchBytes, "this shot needs " + std::to_string(total) + " bytes of workspace; the "
+                                        "allocation-free entry point holds at most " + std::to_string(kScratchBytes) +
+                                        " bytes (use fwi3d_propagate_device)");
+        size_t off = 0;
+        for (int k = 0; k < 7; k++) {
+            saved[k] = *bufs[k];
+            DevBuf b;
+            b.p = sz[k] ? (void*)(ws + off) : nullptr;
+            b.bytes = sz[k];
+            b.borrowed = true;
+            *bufs[k] = b;
+            off += sz[k];
+        }
+        swapped = true;
+        c->use_graphs = false;  // graph instantiation may allocate
+        c->prof = false;        // profiling creates events
+        if (ac)
+            propagate_acoustic(c, adj, d_model, d_src, d_rcv, inj, ext, nullptr, par, g);
+        else
+            propagate_elastic(c, physics == FWI3D_VTI, adj, d_model, d_src, d_rcv, inj, ext, nullptr, par, g);
+        restore();
+        return FWI3D_OK;
+    } catch (const Err& e) {
+        // the stream may still hold work on the workspace: wait before handing the
+        // buffers back, so a later call cannot overlap this one's kernels
+        (void)hipStreamSynchronize(c->stream);
+        restore();
         return fail(c, e);
     }
 }

@@ -106,6 +106,21 @@ int fwi3d_propagate_device(fwi3d_ctx *ctx, int physics, int adj, const float *d_
                            float *d_rcv, const fwi3d_injector *inj, const fwi3d_injector *ext, double *d_grad64,
                            const fwi3d_param *par, int nx, int ny, int nz, float dx, float dy, float dz);
 
+/* Allocation-free variant of fwi3d_propagate_device. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The wavefields, converted
+ * model, flux buffer, taper and injector tables of the shot live in a static 128 MiB
+ * device workspace that each GPU holds while the process runs; the call performs no
+ * hipMalloc / hipFree and replays no graphs. Propagation without a snapshot store or
+ * gradient only (par->sub must be 0; d_grad64 of the allocating entry point has no
+ * counterpart here); a shot whose buffers exceed the workspace is rejected with
+ * FWI3D_EINVAL. Every call on one GPU shares that workspace: calls that overlap on the
+ * same GPU (other contexts, other threads) race and must be serialized by the caller.
+ * Returns after the work has completed; results equal fwi3d_propagate_device's. */
+int fwi3d_propagate_device_noalloc(fwi3d_ctx *ctx, int physics, int adj, const float *d_model, const float *d_src,
+                                   float *d_rcv, const fwi3d_injector *inj, const fwi3d_injector *ext,
+                                   const fwi3d_param *par, int nx, int ny, int nz, float dx, float dy, float dz);
+
 /* Kernel timing with HIP events on the context's stream, for roofline accounting
  * (no counterpart in the reference). enable=1 starts collecting every class below (and
  * resets), enable=2 only whole time steps (class 4: fewer event records in the timed
