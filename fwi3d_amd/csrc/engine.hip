@@ -399,8 +399,11 @@ FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk) {
     }
     F.ychunk = std::max(1, std::min(yc, std::max(nyo, 1)));
     F.nyc = nyo > 0 ? (nyo + F.ychunk - 1) / F.ychunk : 0;
+    // corner tiles: y chunks of a quarter of the others' (at least 4 planes)
+    F.cychunk = std::max(1, std::min(std::max(4, (F.ychunk + 3) / 4), std::max(nyo, 1)));
+    F.cnyc = nyo > 0 ? (nyo + F.cychunk - 1) / F.cychunk : 0;
     P.nfused = fused_middle_tiles(F.ntz, F.ntx) * F.nyc;
-    P.nfedge = fused_edge_tiles(F.ntz, F.ntx) * F.nyc;
+    P.nfedge = fused_corner_tiles(F.ntz, F.ntx) * F.cnyc + fused_side_tiles(F.ntz, F.ntx) * F.nyc;
     EArgs& A = d.ef;
     if (g.ny >= 16) {
         A.org = 0;

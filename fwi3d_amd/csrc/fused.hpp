@@ -34,7 +34,8 @@ constexpr int FTHREADS = FZ * FW;      // 512
 
 struct FusedTiles {
     int ntz, ntx, nyc, ychunk;
-    int ya, yb;  // output planes [ya, yb) = [6, ny-6)
+    int ya, yb;          // output planes [ya, yb) = [6, ny-6)
+    int cnyc, cychunk;   // y chunks of the corner tiles (shorter: they are the costliest)
 };
 
 __host__ __device__ inline int fused_tile_count(int n, int w) {
@@ -72,33 +73,106 @@ __device__ __forceinline__ float d1c(float m2, float m1, float p1, float p2) {
     return D1C1 * (p1 - m1) + D1C2 * (p2 - m2);
 }
 
+// Closure rows with several outputs per row switch (edge tiles). The operands are loaded
+// once into registers (u / b at the 8 points from the face for D2(b), the 6 points for
+// D1), and a lane on closure row r then runs one switch case for all its outputs. D2(b)
+// row r forms c_j = sum_k w_rjk b_k once per u point j (outputs that share a coefficient
+// field share these sums) and then sum_j c_j u_j per output.
+__host__ __device__ constexpr bool d2_uses(int r, int j) {
+    for (int e = 0; e < 132; e++)
+        if (kD2Closure[e].r == r && kD2Closure[e].j == j) return true;
+    return false;
+}
+template <int R, int J, int E>
+__device__ __forceinline__ float d2_cj(float acc, const float (&b)[8]) {
+    if constexpr (E == 132) return acc;
+    else if constexpr (kD2Closure[E].r == R && kD2Closure[E].j == J)
+        return d2_cj<R, J, E + 1>(acc + D2Term<E>::w * b[kD2Closure[E].k], b);
+    else return d2_cj<R, J, E + 1>(acc, b);
+}
+template <int R, int J, int NC>
+__device__ __forceinline__ void d2_row_js(const float (&b)[NC][8], const float (&u)[NC][8], float (&out)[NC]) {
+    if constexpr (J < 8) {
+        if constexpr (d2_uses(R, J)) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[c] += d2_cj<R, J, 0>(-0.0f, b[c]) * u[c][J];
+        }
+        d2_row_js<R, J + 1, NC>(b, u, out);
+    }
+}
+// D2(b) (times h^2) of closure row r in [0, 6) for NC outputs.
+template <int NC>
+__device__ __forceinline__ void d2_rows_n(int r, const float (&b)[NC][8], const float (&u)[NC][8], float (&out)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[c] = -0.0f;
+    switch (r) {
+        case 0: d2_row_js<0, 0, NC>(b, u, out); break;
+        case 1: d2_row_js<1, 0, NC>(b, u, out); break;
+        case 2: d2_row_js<2, 0, NC>(b, u, out); break;
+        case 3: d2_row_js<3, 0, NC>(b, u, out); break;
+        case 4: d2_row_js<4, 0, NC>(b, u, out); break;
+        default: d2_row_js<5, 0, NC>(b, u, out); break;
+    }
+}
+// D1 (times h) of closure row r in [0, 4) for NC outputs; lo as for d1_closure_at.
+template <int NC>
+__device__ __forceinline__ void d1_rows_n(int r, bool lo, const float (&u)[NC][6], float (&out)[NC]) {
+    switch (r) {
+        case 0:
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[c] = d1_row<0>([&](int j) { return u[c][j]; });
+            break;
+        case 1:
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[c] = d1_row<1>([&](int j) { return u[c][j]; });
+            break;
+        case 2:
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[c] = d1_row<2>([&](int j) { return u[c][j]; });
+            break;
+        default:
+#pragma unroll
+            for (int c = 0; c < NC; c++) out[c] = d1_row<3>([&](int j) { return u[c][j]; });
+            break;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[c] = lo ? out[c] : -out[c];
+}
+
 // Tile classes: the tiles away from the x and z faces (the middle tiles of both axes,
 // whose outputs, and the fluxes and stencils they read, are at least 14 points inside
 // those faces) run a body without closure rows and face terms; the others (edge tiles)
-// the full body. Edge tile e of a y chunk: the first and last x columns of tiles (all z
-// tiles), then the first and last z tiles of the x columns between.
-__host__ __device__ inline int fused_edge_tiles(int ntz, int ntx) {
-    return (ntz < 3 || ntx < 3) ? ntz * ntx : ntz * ntx - (ntz - 2) * (ntx - 2);
-}
+// the full body. Edge tiles are the corner tiles (first or last tile along both x and
+// z: closure rows of both axes) and the side tiles (the other tiles of the first and last
+// x columns of tiles, then those of the first and last z tiles).
 __host__ __device__ inline int fused_middle_tiles(int ntz, int ntx) {
     return (ntz < 3 || ntx < 3) ? 0 : (ntz - 2) * (ntx - 2);
 }
-__device__ __forceinline__ void fused_edge_tile(int e, int ntz, int ntx, int& tz, int& tx) {
-    if (ntz < 3 || ntx < 3) {
-        tz = e % ntz; tx = e / ntz;
-    } else if (e < 2 * ntz) {
-        tz = e % ntz; tx = e < ntz ? 0 : ntx - 1;
-    } else {
-        const int f = e - 2 * ntz;
-        tx = 1 + (f >> 1); tz = (f & 1) ? ntz - 1 : 0;
-    }
+__host__ __device__ inline int fused_corner_tiles(int ntz, int ntx) {
+    return (ntz == 1 ? 1 : 2) * (ntx == 1 ? 1 : 2);
+}
+__host__ __device__ inline int fused_side_tiles(int ntz, int ntx) {
+    return (ntx == 1 ? 1 : 2) * (ntz > 2 ? ntz - 2 : 0) + (ntz == 1 ? 1 : 2) * (ntx > 2 ? ntx - 2 : 0);
+}
+// Corner tile c: (tz, tx) packed as tz | tx << 16.
+__device__ __forceinline__ int fused_corner_tile(int c, int ntz, int ntx) {
+    const int nzc = ntz == 1 ? 1 : 2;
+    return ((c % nzc) == 0 ? 0 : ntz - 1) | (((c / nzc) == 0 ? 0 : ntx - 1) << 16);
+}
+// Side tile e: the x-edge columns' tiles between the corners, then the z-edge rows'.
+__device__ __forceinline__ int fused_side_tile(int e, int ntz, int ntx) {
+    const int mz = ntz > 2 ? ntz - 2 : 0, nxs = (ntx == 1 ? 1 : 2) * mz;
+    const int mz1 = mz > 0 ? mz : 1, mx = ntx > 2 ? ntx - 2 : 1, f = e - nxs;
+    const int tz = e < nxs ? 1 + e % mz1 : (f / mx == 0 ? 0 : ntz - 1);
+    const int tx = e < nxs ? (e / mz1 == 0 ? 0 : ntx - 1) : 1 + f % mx;
+    return tz | (tx << 16);
 }
 
-// One workgroup: tile (tzi, txi) of y chunk cy. XE / ZE: the tile may hold x / z closure
-// rows and face points (false only for middle tiles).
+// One workgroup: tile (tzi, txi), output planes [ya, yb). XE / ZE: the tile may hold x / z
+// closure rows and face points (false only for middle tiles).
 template <bool VTI, bool XE, bool ZE>
 __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, const FusedTiles& FT, int tzi, int txi,
-                                                   int cy, FusedSmem<VTI>& S) {
+                                                   int ya, int yb, FusedSmem<VTI>& S) {
     constexpr int NB = VTI ? 3 : 1;
     const Geom& g = A.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz, sx = g.sx, sy = g.sy, n3 = g.n;
@@ -106,7 +180,6 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     int z0, zo0, zo1, x0, xo0, xo1;
     fused_tile(nz, FZ, tzi, FT.ntz, z0, zo0, zo1);
     fused_tile(nx, FW, txi, FT.ntx, x0, xo0, xo1);
-    const int ya = FT.ya + cy * FT.ychunk, yb = min(FT.yb, ya + FT.ychunk);
     const int t = threadIdx.x;
     // Thread -> point of the tile. A z-edge tile holding the closure rows of one z face puts
     // those 6 z rows x 16 x rows on threads 0..95 (one z row per 16 lanes, so the rows'
@@ -149,14 +222,19 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     auto xrow = [&](int k) { return max(0, min(FPX - 1, xb + k * xs)); };
     auto zcol = [&](int k) { return max(0, min(FPZ - 1, zb + k * zs)); };
 
-    // halo role: 1 x-halo point, 2 z-halo point, 3 corner (staging only), 0 none
-    const int role = t < 128 ? 1 : (t < 192 ? 2 : (t < 208 ? 3 : 0));
+    // halo role: 1 x-halo point, 2 z-halo point, 3 corner (staging only), 0 none. The
+    // roles go to threads th < 208; th = t except in tiles whose closure rows sit on the
+    // first waves (z-edge tiles, tiles at the low x face), where th = t + 256 (mod 512)
+    // moves the halo work to the last waves.
+    const bool rshift = zface != 0 || (XE && x0 == 0 && xo1 < nx);
+    const int th = rshift ? (t + 256) & (FTHREADS - 1) : t;
+    const int role = th < 128 ? 1 : (th < 192 ? 2 : (th < 208 ? 3 : 0));
     int hrow = 0, hcol = 0, hoff = col, eoff = col, hx = ix, hz = iz;
     int ca = 0, cb = 1, cc = 2;  // halo components: queued a, queued b, staged-only c
     // an x-halo point's z lane (its own, whatever the thread's own point)
-    const int hzl = t & (FZ - 1), hzc = clz(z0 + hzl);
+    const int hzl = th & (FZ - 1), hzc = clz(z0 + hzl);
     if (role == 1) {
-        const int hr = t >> 5;
+        const int hr = th >> 5;
         hrow = hr < 2 ? hr : FW + hr;
         hcol = hzl + 2;
         hx = hr < 2 ? x0 - 2 + hr : x0 + FW + hr - 2;
@@ -164,7 +242,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
         hoff = clx(hx) * sx + hzc;
         eoff = clx(hr < 2 ? x0 - 4 + hr : x0 + FW + hr) * sx + hzc;
     } else if (role == 2) {
-        const int h = t - 128, zrw = h >> 2, c4 = h & 3;
+        const int h = th - 128, zrw = h >> 2, c4 = h & 3;
         hrow = zrw + 2;
         hcol = c4 < 2 ? c4 : FZ + c4;
         hx = x0 + zrw;
@@ -173,7 +251,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
         eoff = clx(hx) * sx + clz(c4 < 2 ? z0 - 4 + c4 : z0 + FZ + c4);
         ca = 1; cb = 2; cc = 0;
     } else if (role == 3) {
-        const int h = t - 192, cr = h >> 2, c4 = h & 3;
+        const int h = th - 192, cr = h >> 2, c4 = h & 3;
         hrow = cr < 2 ? cr : FW + cr;
         hcol = c4 < 2 ? c4 : FZ + c4;
         hx = cr < 2 ? x0 - 2 + cr : x0 + FW + cr - 2;
@@ -189,7 +267,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     // ---- register queues
     float qx[6], qz[6], qy[7];  // own u_x, u_z at o-2..o+3; u_y at o-2..o+4
     float qb[NB][6];            // own D2 coefficient fields at o-2..o+3
-    float lq, eq, c13q;         // own lambda (VTI: eps, c13) at o+2
+    float lq[3], eq[3], c13q[3];  // own lambda (VTI: eps, c13) at o..o+2
     float tq[5], fyxq[5], fyzq[5];  // T_y, F_yx, F_yz at o-2..o+2
     float txq[3], tzq[3];           // VTI: T_x, T_z at o..o+2
     float rq[4];                    // rho and prev (3) at o
@@ -202,11 +280,11 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     // columns; each halo thread loads the one or two it needs itself
     int eo0 = hoff, eo1 = hoff;
     if (role == 1) {
-        const int hr = t >> 5;
+        const int hr = th >> 5;
         eo0 = clx(hr < 2 ? x0 - 4 + hr : x0 + FW + 2) * sx + hzc;
         eo1 = clx(hr == 0 ? x0 - 3 : x0 + FW + 3) * sx + hzc;
     } else if (role == 2) {
-        const int zrw = (t - 128) >> 2, c4 = t & 3;
+        const int zrw = (th - 128) >> 2, c4 = th & 3;
         eo0 = clx(x0 + zrw) * sx + clz(c4 < 2 ? z0 - 4 + c4 : z0 + FZ + 2) + 2 * n3;
         eo1 = clx(x0 + zrw) * sx + clz(c4 == 0 ? z0 - 3 : z0 + FZ + 3) + 2 * n3;
     }
@@ -220,7 +298,15 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) qy[k] = ld(curr + n3, o0 - 2 + k, col);
-    lq = eq = c13q = 0.f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        lq[k] = ld(lamf, o0 + k, col);
+        eq[k] = VTI ? ld(A.eps, o0 + k, col) : 0.f;
+        c13q[k] = VTI ? ld(A.c13, o0 + k, col) : 0.f;
+    }
+    lq[2] = eq[2] = c13q[2] = 0.f;
+    // taper factors of the own point along z and x (the y factor is read per plane)
+    const float tfz = gld(A.tz2, cz), tfx = gld(A.tx2, min(ix, nx - 1));
 #pragma unroll
     for (int k = 0; k < 5; k++) tq[k] = fyxq[k] = fyzq[k] = 0.f;
 #pragma unroll
@@ -281,12 +367,14 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
         qy[6] = ld(curr + n3, o + 4, col);
 #pragma unroll
         for (int m = 0; m < NB; m++) qb[m][5] = ld(bfld[m], o + 3, col);
-        lq = ld(lamf, o + 2, col);
+        lq[2] = ld(lamf, o + 2, col);
         if (VTI) {
-            eq = ld(A.eps, o + 2, col);
-            c13q = ld(A.c13, o + 2, col);
+            eq[2] = ld(A.eps, o + 2, col);
+            c13q[2] = ld(A.c13, o + 2, col);
         }
+        float tfy = 0.f;
         if (live) {
+            tfy = gld(A.ty2, o);
             rq[0] = ld(rhof, o, col);
 #pragma unroll
             for (int c = 0; c < 3; c++) rq[1 + c] = ld(prev + c * n3, o, col);
@@ -340,7 +428,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
             if (role == 1) {
                 // x-halo point (hx, iz): the D1 along x reaches the ext values, the D1 along
                 // z may use z closures
-                const int hr = t >> 5;
+                const int hr = th >> 5;
                 const float r0 = U(0, hrow - 2 < 0 ? 0 : hrow - 2, hzl + 2), r1 = U(0, hrow - 1 < 0 ? 0 : hrow - 1, hzl + 2);
                 const float r3 = U(0, hrow + 1 >= FPX ? FPX - 1 : hrow + 1, hzl + 2);
                 const float r4 = U(0, hrow + 2 >= FPX ? FPX - 1 : hrow + 2, hzl + 2);
@@ -359,8 +447,16 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     dzu2 = d1c(U(2, hrow, hzl), U(2, hrow, hzl + 1), U(2, hrow, hzl + 3), U(2, hrow, hzl + 4));
                     dzu0 = d1c(U(0, hrow, hzl), U(0, hrow, hzl + 1), U(0, hrow, hzl + 3), U(0, hrow, hzl + 4));
                 } else {
-                    dzu2 = d1_closure_at(hzlo, hzr, [&](int k) { return U(2, hrow, hzcol(k)); });
-                    dzu0 = d1_closure_at(hzlo, hzr, [&](int k) { return U(0, hrow, hzcol(k)); });
+                    float uu[2][6], o[2];
+#pragma unroll
+                    for (int k = 0; k < 6; k++) {
+                        const int zc = hzcol(k);
+                        uu[0][k] = U(2, hrow, zc);
+                        uu[1][k] = U(0, hrow, zc);
+                    }
+                    d1_rows_n<2>(hzr, hzlo, uu, o);
+                    dzu2 = o[0];
+                    dzu0 = o[1];
                 }
                 dzu2 *= ihz;
                 dzu0 *= ihz;
@@ -382,7 +478,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
             } else if (role == 2) {
                 // z-halo point (hx, hz): the D1 along z reaches the ext values, the D1 along x
                 // may use x closures
-                const int zrw = hrow - 2, c4 = t & 3;
+                const int zrw = hrow - 2, c4 = th & 3;
                 const int hxr = hx < nx - 1 - hx ? hx : nx - 1 - hx;
                 const bool hxlo = hx < nx - 1 - hx;
                 const int hxb = max(0, min(FPX - 1, (hxlo ? 0 : nx - 1) - x0 + 2));
@@ -400,8 +496,16 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     dxu0 = d1c(U(0, hrow - 2, hcol), U(0, hrow - 1, hcol), U(0, hrow + 1, hcol), U(0, hrow + 2, hcol));
                     dxu2 = d1c(U(2, hrow - 2, hcol), U(2, hrow - 1, hcol), U(2, hrow + 1, hcol), U(2, hrow + 2, hcol));
                 } else {
-                    dxu0 = d1_closure_at(hxlo, hxr, [&](int k) { return U(0, hxrow(k), hcol); });
-                    dxu2 = d1_closure_at(hxlo, hxr, [&](int k) { return U(2, hxrow(k), hcol); });
+                    float uu[2][6], o[2];
+#pragma unroll
+                    for (int k = 0; k < 6; k++) {
+                        const int xc = hxrow(k);
+                        uu[0][k] = U(0, xc, hcol);
+                        uu[1][k] = U(2, xc, hcol);
+                    }
+                    d1_rows_n<2>(hxr, hxlo, uu, o);
+                    dxu0 = o[0];
+                    dxu2 = o[1];
                 }
                 dxu0 *= ihx;
                 dxu2 *= ihx;
@@ -422,24 +526,46 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
         // update of plane o reads the queues
         {
             auto UL = [&](int c, int r, int z) { return S.u[(o + 2) & 3][c][r][z]; };
-            auto dX = [&](int c) {  // D_x (times h) at the own point
-                if (!XE || xr >= 4) return d1c(UL(c, lx, lz + 2), UL(c, lx + 1, lz + 2), UL(c, lx + 3, lz + 2), UL(c, lx + 4, lz + 2));
-                return d1_closure_at(xlo, xr, [&](int k) { return UL(c, xrow(k), lz + 2); });
-            };
-            auto dZ = [&](int c) {
-                if (!ZE || zr >= 4) return d1c(UL(c, lx + 2, lz), UL(c, lx + 2, lz + 1), UL(c, lx + 2, lz + 3), UL(c, lx + 2, lz + 4));
-                return d1_closure_at(zlo, zr, [&](int k) { return UL(c, lx + 2, zcol(k)); });
-            };
-            const float dxu0 = dX(0) * ihx, dxu1 = dX(1) * ihx;
-            const float dzu1 = dZ(1) * ihz, dzu2 = dZ(2) * ihz;
+            // D_x of u_x, u_y and D_z of u_y, u_z (times h) at the own point
+            float ox2[2], oz2[2];
+            if (!XE || xr >= 4) {
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    ox2[c] = d1c(UL(c, lx, lz + 2), UL(c, lx + 1, lz + 2), UL(c, lx + 3, lz + 2), UL(c, lx + 4, lz + 2));
+            } else {
+                float uu[2][6];
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    const int xc = xrow(k);
+                    uu[0][k] = UL(0, xc, lz + 2);
+                    uu[1][k] = UL(1, xc, lz + 2);
+                }
+                d1_rows_n<2>(xr, xlo, uu, ox2);
+            }
+            if (!ZE || zr >= 4) {
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    oz2[c] = d1c(UL(c + 1, lx + 2, lz), UL(c + 1, lx + 2, lz + 1), UL(c + 1, lx + 2, lz + 3), UL(c + 1, lx + 2, lz + 4));
+            } else {
+                float uu[2][6];
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    const int zc = zcol(k);
+                    uu[0][k] = UL(1, lx + 2, zc);
+                    uu[1][k] = UL(2, lx + 2, zc);
+                }
+                d1_rows_n<2>(zr, zlo, uu, oz2);
+            }
+            const float dxu0 = ox2[0] * ihx, dxu1 = ox2[1] * ihx;
+            const float dzu1 = oz2[0] * ihz, dzu2 = oz2[1] * ihz;
             const float dyu1 = d1c(qy[2], qy[3], qy[5], qy[6]) * ihy;
             const float mu = qb[0][4];
-            const float lam = lq;
+            const float lam = lq[2];
             float Ty;
             if (!VTI) {
                 Ty = lam * (dxu0 + dyu1 + dzu2);
             } else {
-                const float e2 = 1.f + 2.f * eq, c13 = c13q, c66 = qb[1][4];
+                const float e2 = 1.f + 2.f * eq[2], c13 = c13q[2], c66 = qb[1][4];
                 const float c11c = e2 * lam, c12 = e2 * (lam + 2.f * mu) - 2.f * c66;
                 Ty = c11c * dyu1 + c12 * dxu0 + c13 * dzu2;
                 txq[0] = txq[1]; txq[1] = txq[2];
@@ -463,57 +589,113 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
             if (outp) {
                 const float ih2x = ihx * ihx, ih2y = ihy * ihy, ih2z = ihz * ihz;
                 const int p = o * sy + ix * sx + iz;
-                auto d2x = [&](int c, int m) -> float {
-                    if (!XE || xr >= 6)
-                        return d2_interior(U(c, lx, lz + 2), U(c, lx + 1, lz + 2), U(c, lx + 2, lz + 2), U(c, lx + 3, lz + 2),
-                                           U(c, lx + 4, lz + 2), Bf(m, lx, lz + 2), Bf(m, lx + 1, lz + 2), Bf(m, lx + 2, lz + 2),
-                                           Bf(m, lx + 3, lz + 2), Bf(m, lx + 4, lz + 2));
-                    return d2_rows(xr, [&](int j) { return U(c, xrow(j), lz + 2); }, [&](int k) { return Bf(m, xrow(k), lz + 2); });
-                };
-                auto d2z = [&](int c) -> float {  // b = mu along z for every component
-                    if (!ZE || zr >= 6)
-                        return d2_interior(U(c, lx + 2, lz), U(c, lx + 2, lz + 1), U(c, lx + 2, lz + 2), U(c, lx + 2, lz + 3),
-                                           U(c, lx + 2, lz + 4), Bf(0, lx + 2, lz), Bf(0, lx + 2, lz + 1), Bf(0, lx + 2, lz + 2),
-                                           Bf(0, lx + 2, lz + 3), Bf(0, lx + 2, lz + 4));
-                    return d2_rows(zr, [&](int j) { return U(c, lx + 2, zcol(j)); }, [&](int k) { return Bf(0, lx + 2, zcol(k)); });
-                };
+                // D2(b) along x and z of the three components (times h^2); along z the
+                // coefficient field is mu for every component
+                float d2x[3], d2z[3];
+                if (!XE || xr >= 6) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const int m = d2_bfield<VTI>(c, 0);
+                        d2x[c] = d2_interior(U(c, lx, lz + 2), U(c, lx + 1, lz + 2), U(c, lx + 2, lz + 2), U(c, lx + 3, lz + 2),
+                                             U(c, lx + 4, lz + 2), Bf(m, lx, lz + 2), Bf(m, lx + 1, lz + 2), Bf(m, lx + 2, lz + 2),
+                                             Bf(m, lx + 3, lz + 2), Bf(m, lx + 4, lz + 2));
+                    }
+                } else {
+                    float uu[3][8], bb[3][8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        const int xc = xrow(k);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            uu[c][k] = U(c, xc, lz + 2);
+                            bb[c][k] = Bf(d2_bfield<VTI>(c, 0), xc, lz + 2);
+                        }
+                    }
+                    d2_rows_n<3>(xr, bb, uu, d2x);
+                }
+                if (!ZE || zr >= 6) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        d2z[c] = d2_interior(U(c, lx + 2, lz), U(c, lx + 2, lz + 1), U(c, lx + 2, lz + 2), U(c, lx + 2, lz + 3),
+                                             U(c, lx + 2, lz + 4), Bf(0, lx + 2, lz), Bf(0, lx + 2, lz + 1), Bf(0, lx + 2, lz + 2),
+                                             Bf(0, lx + 2, lz + 3), Bf(0, lx + 2, lz + 4));
+                } else {
+                    float uu[3][8], bb[3][8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        const int zc = zcol(k);
+                        const float bk = Bf(0, lx + 2, zc);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            uu[c][k] = U(c, lx + 2, zc);
+                            bb[c][k] = bk;
+                        }
+                    }
+                    d2_rows_n<3>(zr, bb, uu, d2z);
+                }
                 auto qc = [&](int c, int k) { return c == 0 ? qx[k] : (c == 1 ? qy[k] : qz[k]); };
                 auto d2y = [&](int c, int m) {
                     return d2_interior(qc(c, 0), qc(c, 1), qc(c, 2), qc(c, 3), qc(c, 4), qb[m][0], qb[m][1], qb[m][2],
                                        qb[m][3], qb[m][4]);
                 };
-                // outer D1 of the staged flux planes at the own point (times h)
-                auto ox = [&](const float(*P)[FZ]) {
-                    if (!XE || xr >= 4) return d1c(P[lx][lz], P[lx + 1][lz], P[lx + 3][lz], P[lx + 4][lz]);
-                    return d1_closure_at(xlo, xr, [&](int k) { return P[xrow(k)][lz]; });
-                };
-                auto oz = [&](const float(*P)[FPZ]) {
-                    if (!ZE || zr >= 4) return d1c(P[lx][lz], P[lx][lz + 1], P[lx][lz + 3], P[lx][lz + 4]);
-                    return d1_closure_at(zlo, zr, [&](int k) { return P[lx][zcol(k)]; });
-                };
+                // outer D1 (times h) at the own point of the staged flux planes: along x
+                // T / T_x, F_xy, F_xz (components 0, 1, 2), along z F_zx, F_zy, T / T_z
+                float ox[3], oz[3];
+                {
+                    const float(*PX[3])[FZ] = {S.tx[fb], S.fxy[fb], S.fxz[fb]};
+                    const float(*PZ[3])[FPZ] = {S.fzx[fb], S.fzy[fb], S.tz[fb]};
+                    if (!XE || xr >= 4) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            ox[c] = d1c(PX[c][lx][lz], PX[c][lx + 1][lz], PX[c][lx + 3][lz], PX[c][lx + 4][lz]);
+                    } else {
+                        float ff[3][6];
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            const int xc = xrow(k);
+#pragma unroll
+                            for (int c = 0; c < 3; c++) ff[c][k] = PX[c][xc][lz];
+                        }
+                        d1_rows_n<3>(xr, xlo, ff, ox);
+                    }
+                    if (!ZE || zr >= 4) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+                            oz[c] = d1c(PZ[c][lx][lz], PZ[c][lx][lz + 1], PZ[c][lx][lz + 3], PZ[c][lx][lz + 4]);
+                    } else {
+                        float ff[3][6];
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            const int zc = zcol(k);
+#pragma unroll
+                            for (int c = 0; c < 3; c++) ff[c][k] = PZ[c][lx][zc];
+                        }
+                        d1_rows_n<3>(zr, zlo, ff, oz);
+                    }
+                }
                 auto oy = [&](const float* q) { return d1c(q[0], q[1], q[3], q[4]); };
                 float acc[3], pv[3];
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    float a = (d2_factor<VTI>(c, 0) * ih2x) * d2x(c, d2_bfield<VTI>(c, 0));
+                    float a = (d2_factor<VTI>(c, 0) * ih2x) * d2x[c];
                     a = a + (d2_factor<VTI>(c, 1) * ih2y) * d2y(c, d2_bfield<VTI>(c, 1));
-                    a = a + (d2_factor<VTI>(c, 2) * ih2z) * d2z(c);
+                    a = a + (d2_factor<VTI>(c, 2) * ih2z) * d2z[c];
                     if (c == 0) {
-                        a = a + ox(S.tx[fb]) * ihx;
-                        a = a + oy(fyxq) * ihy;     // D_y F_yx
-                        a = a + oz(S.fzx[fb]) * ihy;  // D_z F_zx (spacing dy as in the reference)
+                        a = a + ox[0] * ihx;
+                        a = a + oy(fyxq) * ihy;  // D_y F_yx
+                        a = a + oz[0] * ihy;     // D_z F_zx (spacing dy as in the reference)
                     } else if (c == 1) {
-                        a = a + oy(tq) * ihy;       // D_y T_y
-                        a = a + ox(S.fxy[fb]) * ihx;  // D_x F_xy
-                        a = a + oz(S.fzy[fb]) * ihz;  // D_z F_zy
+                        a = a + oy(tq) * ihy;    // D_y T_y
+                        a = a + ox[1] * ihx;     // D_x F_xy
+                        a = a + oz[1] * ihz;     // D_z F_zy
                     } else {
-                        a = a + oz(S.tz[fb]) * ihz;
-                        a = a + ox(S.fxz[fb]) * ihx;  // D_x F_xz
-                        a = a + oy(fyzq) * ihy;     // D_y F_yz
+                        a = a + oz[2] * ihz;
+                        a = a + ox[2] * ihx;     // D_x F_xz
+                        a = a + oy(fyzq) * ihy;  // D_y F_yz
                     }
                     acc[c] = a;
                     pv[c] = rq[1 + c];
-                    if (A.prev_tapered) pv[c] = taper_apply(pv[c], A.tz2, A.tx2, A.ty2, ix, o, iz);
+                    if (A.prev_tapered) pv[c] = ((pv[c] * tfz) * tfx) * tfy;  // taper_apply's order
                 }
                 const float rho = rq[0];
                 float sc[3] = {0.f, 0.f, 0.f};
@@ -523,24 +705,48 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     // face order (top, bottom, left, right), and the absorbing scaling. D1 of u
                     // at the point (divided by h) and the outward S along x / z (times h):
                     float dX[3], dY[3], dZ[3], sX[3], sZ[3];
+                    {
+                        // the 6 values from the x face and from the z face of each component
+                        float ux[3][6], uz[3][6];
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        dX[c] = (!XE || xr >= 4 ? d1c(U(c, lx, lz + 2), U(c, lx + 1, lz + 2), U(c, lx + 3, lz + 2), U(c, lx + 4, lz + 2))
-                                                : d1_closure_at(xlo, xr, [&](int k) { return U(c, xrow(k), lz + 2); })) * ihx;
-                        dZ[c] = (!ZE || zr >= 4 ? d1c(U(c, lx + 2, lz), U(c, lx + 2, lz + 1), U(c, lx + 2, lz + 3), U(c, lx + 2, lz + 4))
-                                                : d1_closure_at(zlo, zr, [&](int k) { return U(c, lx + 2, zcol(k)); })) * ihz;
-                        dY[c] = d1c(qc(c, 0), qc(c, 1), qc(c, 3), qc(c, 4)) * ihy;
-                        sX[c] = c_sat_s[0] * U(c, xrow(0), lz + 2) + c_sat_s[1] * U(c, xrow(1), lz + 2) +
-                                c_sat_s[2] * U(c, xrow(2), lz + 2) + c_sat_s[3] * U(c, xrow(3), lz + 2);
-                        sZ[c] = c_sat_s[0] * U(c, lx + 2, zcol(0)) + c_sat_s[1] * U(c, lx + 2, zcol(1)) +
-                                c_sat_s[2] * U(c, lx + 2, zcol(2)) + c_sat_s[3] * U(c, lx + 2, zcol(3));
+                        for (int k = 0; k < 6; k++) {
+                            const int xc = xrow(k), zc = zcol(k);
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                ux[c][k] = U(c, xc, lz + 2);
+                                uz[c][k] = U(c, lx + 2, zc);
+                            }
+                        }
+                        if (!XE || xr >= 4) {
+#pragma unroll
+                            for (int c = 0; c < 3; c++)
+                                dX[c] = d1c(U(c, lx, lz + 2), U(c, lx + 1, lz + 2), U(c, lx + 3, lz + 2), U(c, lx + 4, lz + 2));
+                        } else {
+                            d1_rows_n<3>(xr, xlo, ux, dX);
+                        }
+                        if (!ZE || zr >= 4) {
+#pragma unroll
+                            for (int c = 0; c < 3; c++)
+                                dZ[c] = d1c(U(c, lx + 2, lz), U(c, lx + 2, lz + 1), U(c, lx + 2, lz + 3), U(c, lx + 2, lz + 4));
+                        } else {
+                            d1_rows_n<3>(zr, zlo, uz, dZ);
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            dX[c] *= ihx;
+                            dZ[c] *= ihz;
+                            dY[c] = d1c(qc(c, 0), qc(c, 1), qc(c, 3), qc(c, 4)) * ihy;
+                            sX[c] = c_sat_s[0] * ux[c][0] + c_sat_s[1] * ux[c][1] + c_sat_s[2] * ux[c][2] + c_sat_s[3] * ux[c][3];
+                            sZ[c] = c_sat_s[0] * uz[c][0] + c_sat_s[1] * uz[c][1] + c_sat_s[2] * uz[c][2] + c_sat_s[3] * uz[c][3];
+                        }
                     }
-                    const float lam = gld(A.lam, p), mu = gld(A.mu, p);
+                    // the model at the point, from the queues (plane o)
+                    const float lam = lq[0], mu = qb[0][2];
                     float e2 = 1.f, c13 = 0.f, c66 = mu;
                     if (VTI) {
-                        e2 = 1.f + 2.f * gld(A.eps, p);
-                        c13 = gld(A.c13, p);
-                        c66 = gld(A.c66, p);
+                        e2 = 1.f + 2.f * eq[0];
+                        c13 = c13q[0];
+                        c66 = qb[1][2];
                     }
                     const int bcz0 = A.bc[0], bcz1 = A.bc[1], bcx0 = A.bc[2], bcx1 = A.bc[3];
                     for (int f = 0; f < 4; f++) {
@@ -586,7 +792,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                 for (int c = 0; c < 3; c++) {
                     float nv = A.dt2 * acc[c] / rho + 2.f * qc(c, 2) - pv[c];
                     if (absorb) nv = nv / (1.f + sc[c]);
-                    gst(A.next, c * n3 + p, taper_apply(nv, A.tz2, A.tx2, A.ty2, ix, o, iz));
+                    gst(A.next, c * n3 + p, ((nv * tfz) * tfx) * tfy);
                 }
             }
         }
@@ -615,6 +821,12 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
         }
 #pragma unroll
         for (int k = 0; k < 6; k++) qy[k] = qy[k + 1];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            lq[k] = lq[k + 1];
+            eq[k] = eq[k + 1];
+            c13q[k] = c13q[k + 1];
+        }
         hm[0] = hm[1];
         hl[0] = hl[1];
         ve[0][0] = ve[1][0];

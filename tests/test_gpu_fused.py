@@ -44,7 +44,7 @@ def every_point(eng, physics, shape, case, nt=40, fchunk=5):
     try:
         got = eng.propagate(physics, model, src, inj, ext, par, d)
     finally:
-        eng.tune(fchunk=0)
+        eng.tune(fused=False, fchunk=0)
     return got, (model, src, sloc, rloc, d, dt, cfg)
 
 
@@ -87,7 +87,7 @@ def test_fused_equals_unfused(eng, physics):
             try:
                 out[fused, case] = eng.propagate(physics, model, src, inj, ext, par, d)
             finally:
-                eng.tune(fused=True)
+                eng.tune(fused=False)
     for case in ("mixed", "free+absorb+taper"):
         e = rel(out[True, case], out[False, case])
         print(f"  {case}: rel L2 fused vs unfused {e:.2e}")
