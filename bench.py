@@ -82,10 +82,41 @@ def cpu_baseline(shot, seconds=6.0, gradient=True):
                       f"(oracle FP32 C++/OpenMP); median of 3 runs after one warm-up"}
 
 
+# fwi3d_tune settings of every engine the workloads create (--unfused)
+ENGINE_TUNE = {}
+
+# profiled kernel classes of the elastic time step (fwi3d_profile_read), by step form
+KERNEL_CLASSES = {
+    True: ((0, "elastic_fused"), (1, "elastic_flux_yfaces"), (3, "elastic_yslabs")),
+    False: ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"), (3, "elastic_zslab")),
+}
+STEP_KERNEL = {
+    True: "elastic time step = elastic_fused_batch (middle and edge tiles, planes [6, ny-6)) + elastic_flux_batch "
+          "(the 8 planes at each y face) + elastic_yslabs_ext (y-face slabs, edges, receivers) launches (HIP events "
+          "around each step on the engine stream)",
+    False: "elastic time step = elastic_flux + elastic_interior + elastic_boundary + elastic_zslab launches "
+           "(HIP events around each step on the engine stream)",
+}
+
+
+def new_engine(F):
+    eng = F.Engine(0)
+    if ENGINE_TUNE:
+        eng.tune(**ENGINE_TUNE)
+    return eng
+
+
+def traffic_profile(n):
+    """The committed rocprofv3 --pmc summary of the elastic time step at n^3 for the
+    step form in use (fused by default, --unfused)."""
+    form = "unfused" if ENGINE_TUNE.get("fused") is False else "fused"
+    return os.path.join("profiles", f"elastic_{form}_step_{n}cube_traffic_pmc.md")
+
+
 def measured_traffic(n):
     """HBM bytes per elastic time step (scaled FETCH_SIZE + WRITE_SIZE of the step kernels)
     from the committed rocprofv3 --pmc summary of the same workload, or None."""
-    path = os.path.join(ROOT, "profiles", f"elastic_{n}cube_step_traffic_pmc.md")
+    path = os.path.join(ROOT, traffic_profile(n))
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -119,7 +150,7 @@ def run_lbfgs(args, torch, F, W):
     start = W.config_b(n=n, nt=nt, anomaly=False)
     shape = true.shape
     dev = torch.device("cuda:0")
-    eng = F.Engine(0)
+    eng = new_engine(F)
     geo = F.Shot(shape, true.d, (0.0, 0.0, 0.0), nt, true.dt, true.src_loc[0], true.rcv_loc)
     opkw = dict(nmodels=3, dt=true.dt, sub=true.sub, mt=1, bc=true.bc, taper=true.taper,
                 taper_strength=true.taper_strength)
@@ -200,7 +231,7 @@ def run_batch(args, torch, F, W, emit=True):
     split = (11, 11, 10)
     shots = [W.config_c_shot(g, k % 12, nt=nt) for g in range(3) for k in range(split[g])]
     dev = torch.device("cuda:0")
-    eng = F.Engine(0)
+    eng = new_engine(F)
     bufs = []
     for s in shots:
         inj = F.Injector(1, s.src_loc, s.shape, s.d)
@@ -294,7 +325,7 @@ def run_acoustic(args, torch, F, W):
     shape = shot.shape
     N = float(np.prod(shape))
     dev = torch.device("cuda:0")
-    eng = F.Engine(0)
+    eng = new_engine(F)
     tiles = eng.autotune("acoustic", shape, steps=20)
     geo = F.Shot(shape, shot.d, (0.0, 0.0, 0.0), nt, shot.dt, shot.src_loc[0], shot.rcv_loc)
     par = lambda: F.op_param(nmodels=2, dt=shot.dt, sub=0, bc=shot.bc, taper=shot.taper,
@@ -459,7 +490,7 @@ def run_serve(args, torch, F, W, emit=True):
                               grad.data_ptr(), data.data_ptr())
 
     # (1) plain, one request at a time (a small warm-up request first, untimed)
-    eng = F.Engine(0)
+    eng = new_engine(F)
     warm = serve_inputs(torch, F, W, W.Request(-1, "gradient", "elastic", 40, 50, 0.0008, 1), dev)
     plain_one(eng, warm, warm["data"], warm["grad"])
     plain_out, lat_plain = [], []
@@ -486,7 +517,7 @@ def run_serve(args, torch, F, W, emit=True):
     budget = int(min(args.hbm_budget_gib * 2**30, free - 16 * 2**30))
 
     def serve(arrivals):
-        eng = F.Engine(0)
+        eng = new_engine(F)
         srv = F.Server(eng, budget, args.max_active)
         for q in inputs:
             if "grad" in q:
@@ -598,7 +629,7 @@ def config_d_gradients(torch, F, W, n=256, nt=3000):
     obs_model = W.config_b(n=n, nt=nt, anomaly=False)
     shape = true.shape
     dev = torch.device("cuda:0")
-    eng = F.Engine(0)
+    eng = new_engine(F)
     geo = F.Shot(shape, true.d, (0.0, 0.0, 0.0), nt, true.dt, true.src_loc[0], true.rcv_loc)
     opkw = dict(nmodels=3, dt=true.dt, sub=true.sub, mt=1, bc=true.bc, taper=true.taper,
                 taper_strength=true.taper_strength)
@@ -643,9 +674,9 @@ def main():
                     help="skip the mixed-precision 256^3 gradients, the batched VTI comparison and the reduced "
                          "serving stream that the default run adds after the headline")
     ap.add_argument("--snap", default="f32", choices=("f32", "bf16", "f16"), help="snapshot precision")
-    ap.add_argument("--fused", action="store_true",
-                    help="elastic / VTI steps with the fused kernel (fwi3d_tune FWI3D_TUNE_FUSED) instead of the flux, "
-                         "interior, slab and edge kernels")
+    ap.add_argument("--unfused", action="store_true",
+                    help="elastic / VTI steps with the flux, interior, slab and edge kernels (fwi3d_tune "
+                         "FWI3D_TUNE_FUSED = 0) instead of the fused kernel")
     ap.add_argument("--workload", default="gradient", choices=("gradient", "acoustic", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); acoustic: 100^3 acoustic forward modelling; "
                          "lbfgs: 256^3 mixed-precision l-BFGS FWI; batch: 32 VTI shots on 3 grids, batched vs one at "
@@ -657,6 +688,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates the HBM traffic counters)")
     args = ap.parse_args()
+    if args.unfused:
+        ENGINE_TUNE["fused"] = False
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (shots would shard one per GPU); "
               "use --gpus 1", file=sys.stderr)
@@ -689,9 +722,8 @@ def main():
     N = nx * ny * nz
     nt = shot.nt
     dev = torch.device("cuda:0")
-    eng = F.Engine(0)
-    if args.fused:
-        eng.tune(fused=True)
+    eng = new_engine(F)
+
     geo = F.Shot(shape, shot.d, (0.0, 0.0, 0.0), nt, shot.dt, shot.src_loc[0], shot.rcv_loc)
     # the reference's param for this shot: raw (vp, vs, rho), explosive moment tensor,
     # propagation dt = data dt (sinc resampling still runs), imaging every 4 steps
@@ -727,7 +759,7 @@ def main():
     shot_gradient()
     torch.cuda.synchronize()
     kern = {name: eng.profile_read(k)
-            for k, name in ((0, "elastic_interior"), (1, "elastic_flux"), (2, "elastic_boundary"), (3, "elastic_zslab"))}
+            for k, name in KERNEL_CLASSES[not args.unfused]}
     eng.profile(False)
     for _ in range(max(0, args.warmup - 1)):
         shot_gradient()
@@ -792,9 +824,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": f"profiles/elastic_{args.n}cube_step_traffic_pmc.md" if traffic else None,
-            "kernel": "elastic time step = elastic_flux + elastic_interior + elastic_boundary + elastic_zslab launches "
-                      "(HIP events around each step on the engine stream)",
+            "traffic_source": traffic_profile(args.n) if traffic else None,
+            "kernel": STEP_KERNEL[not args.unfused],
             "step_us": round(t_pair * 1e6, 2),
             "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
             "cells_per_step": N,

@@ -316,8 +316,8 @@ class Engine:
 
     def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None):
         """Tiling of the time-step kernels, graph replay of time loops, and the elastic /
-        VTI step form (fused: the fused kernel over the planes 6 cells inside the y faces;
-        fchunk: its y planes per workgroup, 0 by grid size) (fwi3d_tune). Results do not
+        VTI step form (fused, the default: the fused kernel over the planes 6 cells inside
+        the y faces; fchunk: its y planes per workgroup, 0 by grid size) (fwi3d_tune). Results do not
         depend on the tiling or on graph replay; the two elastic forms agree to rounding."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")

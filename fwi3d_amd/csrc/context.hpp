@@ -112,8 +112,8 @@ struct fwi3d_ctx {
     // (fwi3d_tune FWI3D_TUNE_GRAPHS); the counter the graph's kernels read the step from
     bool use_graphs = true;
     // elastic / VTI steps: the fused kernel over the planes [6, ny-6) (fwi3d_tune
-    // FWI3D_TUNE_FUSED, default off) and its y chunk (FWI3D_TUNE_FCHUNK, 0: by grid size)
-    bool use_fused = false;
+    // FWI3D_TUNE_FUSED, default on) and its y chunk (FWI3D_TUNE_FCHUNK, 0: by grid size)
+    bool use_fused = true;
     int fused_chunk = 0;
     fwi3d::DevBuf stepctr;
     std::vector<hipGraphExec_t> graph_execs;  // instantiated in the current call, freed after it

@@ -76,6 +76,11 @@ class Shot:
     taper_strength: float = 0.05
     meta: dict = field(default_factory=dict)
 
+    def __post_init__(self):
+        # C order: device entry points take bare pointers ([param][y][x][z], z fastest)
+        self.model = np.ascontiguousarray(self.model)
+        self.raw = np.ascontiguousarray(self.raw)
+
     @property
     def shape(self):
         return self.model.shape[-3:]

@@ -136,7 +136,7 @@ int fwi3d_propagate_device_noalloc(fwi3d_ctx *ctx, int physics, int adj, const f
  * FWI3D_TUNE_GRAPHS (default 1): time loops with no host work between steps (forward
  * propagation without snapshots, adjoint propagation without a gradient) replay a
  * captured hipGraph of 16 steps instead of launching each kernel.
- * FWI3D_TUNE_FUSED (default 0): elastic / VTI steps update the planes 6 cells inside the
+ * FWI3D_TUNE_FUSED (default 1): elastic / VTI steps update the planes 6 cells inside the
  * y faces with the fused kernel (stresses and fluxes formed on chip; 0: the flux pass,
  * interior, slab and edge kernels over the whole grid). FWI3D_TUNE_FCHUNK: y planes per
  * workgroup of the fused kernel (0: chosen from the grid size). Results agree to

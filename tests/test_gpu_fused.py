@@ -1,6 +1,7 @@
-"""GPU parity of the fused elastic / VTI time step (fwi3d_amd/csrc/fused.hpp) against the
-oracle, point by point: one launch updates the planes 6 cells inside the y faces with the
-stresses and fluxes of nl_we_op_e / nl_we_op_vti::propagate (src/cpp/we_op.cpp:646-836,
+"""GPU parity against the oracle, point by point, of the fused elastic / VTI time step
+(fwi3d_amd/csrc/fused.hpp, the default form) and of the flux / interior / slab / edge
+form. The fused launch updates the planes 6 cells inside the y faces with the stresses
+and fluxes of nl_we_op_e / nl_we_op_vti::propagate (src/cpp/we_op.cpp:646-836,
 1290-1497) formed on chip, including the x / z closure rows, the traction SATs of the x
 and z faces and their absorbing scaling.
 
@@ -44,7 +45,7 @@ def every_point(eng, physics, shape, case, nt=40, fchunk=5):
     try:
         got = eng.propagate(physics, model, src, inj, ext, par, d)
     finally:
-        eng.tune(fused=False, fchunk=0)
+        eng.tune(fchunk=0)
     return got, (model, src, sloc, rloc, d, dt, cfg)
 
 
@@ -71,6 +72,35 @@ def test_fused_every_point_matches_oracle(eng, physics, shape, case):
 
 
 @pytest.mark.parametrize("physics", ["elastic", "vti"])
+@pytest.mark.parametrize("shape,case", [((30, 45, 70), "mixed"), ((17, 16, 33), "absorb-all")])
+def test_unfused_every_point_matches_oracle(eng, physics, shape, case):
+    """The flux / interior / slab / edge form (fwi3d_tune FWI3D_TUNE_FUSED = 0) at every
+    point against the oracle."""
+    nt = 40
+    print(f"[unfused {physics} every point] shape={shape} case={case} nt={nt}", flush=True)
+    model, src, sloc, _, d, dt = small_shot(physics, 1, shape=shape, nt=nt)
+    ny, nx, nz = shape
+    h = d[0]
+    iy, ix, iz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    rloc = np.stack([ix.ravel() * h, iy.ravel() * h, iz.ravel() * h], axis=1).astype(np.float32)
+    cfg = CASES[case]
+    inj, ext = F.Injector(1, sloc, shape, d), F.Injector(0, rloc, shape, d)
+    par = F.make_param(nt, dt, bc=cfg["bc"], taper=cfg["taper"])
+    eng.tune(fused=False)
+    try:
+        got = eng.propagate(physics, model, src, inj, ext, par, d)
+    finally:
+        eng.tune(fused=True)
+    oinj, oext = O.Injector(1, sloc, shape, d, (0, 0, 0)), O.Injector(0, rloc, shape, d, (0, 0, 0))
+    opar = O.make_par(nt, 0, dt, bc=cfg["bc"], taper=cfg["taper"])
+    want, _ = O.propagate(physics, model, src, oinj, oext, opar, d, s="f")
+    e = rel(got, want)
+    last = np.abs(got[:, :, -1] - want[:, :, -1]).max() / np.abs(want[:, :, -1]).max()
+    print(f"  rel L2 vs oracle f32 {e:.2e}, worst point of the last sample {last:.2e}")
+    assert np.isfinite(got).all() and e <= TRACE_TOL and last <= 10 * TRACE_TOL
+
+
+@pytest.mark.parametrize("physics", ["elastic", "vti"])
 def test_fused_equals_unfused(eng, physics):
     """The fused step and the flux / interior / slab / edge kernels compute the same
     field to rounding (a larger grid than the every-point cases, default y chunks)."""
@@ -87,7 +117,7 @@ def test_fused_equals_unfused(eng, physics):
             try:
                 out[fused, case] = eng.propagate(physics, model, src, inj, ext, par, d)
             finally:
-                eng.tune(fused=False)
+                eng.tune(fused=True)
     for case in ("mixed", "free+absorb+taper"):
         e = rel(out[True, case], out[False, case])
         print(f"  {case}: rel L2 fused vs unfused {e:.2e}")
