@@ -177,26 +177,34 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
     const Geom& g = A.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz, sx = g.sx, sy = g.sy, n3 = g.n;
     const float ihx = 1.0f / g.dx, ihy = 1.0f / g.dy, ihz = 1.0f / g.dz;
+    const float idt = 1.0f / A.dt;
     int z0, zo0, zo1, x0, xo0, xo1;
     fused_tile(nz, FZ, tzi, FT.ntz, z0, zo0, zo1);
     fused_tile(nx, FW, txi, FT.ntx, x0, xo0, xo1);
     const int t = threadIdx.x;
     // Thread -> point of the tile. A z-edge tile holding the closure rows of one z face puts
-    // those 6 z rows x 16 x rows on threads 0..95 (one z row per 16 lanes, so the rows'
-    // closures and face terms run row-uniformly) and its other 26 z rows x 16 x rows on
-    // the rest; every other tile runs 32 z lanes per x row (a wave holds two x rows).
+    // each of those 6 z rows (x 16 x rows) on a 16-lane group, so a row's closures run
+    // row-uniformly: rows 1-3 from the face on the first wave, rows 4-5 on the second, the
+    // face row (traction SAT) on the third; the other 26 z rows x 16 x rows fill the other
+    // groups. Every other tile runs 32 z lanes per x row (a wave holds two x rows).
     const int zface = !ZE ? 0 : ((zo0 == 0) == (zo1 == nz) ? 0 : (zo0 == 0 ? 1 : 2));  // 1 low, 2 high
     int lz, lx;
     if (zface == 0) {
         lz = t & (FZ - 1);
         lx = t >> 5;
-    } else if (t < 6 * FW) {
-        lx = t & (FW - 1);
-        lz = zface == 1 ? (t >> 4) : FZ - 6 + (t >> 4);
     } else {
-        const int u = t - 6 * FW;
-        lx = u / (FZ - 6);
-        lz = zface == 1 ? 6 + u % (FZ - 6) : u % (FZ - 6);
+        const int grp = t >> 4;
+        const int crow = grp < 3 ? grp + 1 : (grp == 4 ? 4 : (grp == 5 ? 5 : (grp == 8 ? 0 : -1)));
+        if (crow >= 0) {
+            lx = t & (FW - 1);
+            lz = zface == 1 ? crow : FZ - 1 - crow;
+        } else {
+            // interior groups before this one (the closure groups are 0-2, 4, 5 and 8)
+            const int nb = grp - min(grp, 3) - (grp > 4) - (grp > 5) - (grp > 8);
+            const int u = nb * 16 + (t & 15);
+            lx = u / (FZ - 6);
+            lz = zface == 1 ? 6 + u % (FZ - 6) : u % (FZ - 6);
+        }
     }
     const int ix = x0 + lx, iz = z0 + lz;
     const int cz = min(iz, nz - 1);
@@ -697,7 +705,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     pv[c] = rq[1 + c];
                     if (A.prev_tapered) pv[c] = ((pv[c] * tfz) * tfx) * tfy;  // taper_apply's order
                 }
-                const float rho = rq[0];
+                const float rho = rq[0], irho = 1.0f / rho;
                 float sc[3] = {0.f, 0.f, 0.f};
                 bool absorb = false;
                 if ((XE && (ix == 0 || ix == nx - 1)) || (ZE && (iz == 0 || iz == nz - 1))) {
@@ -765,8 +773,9 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                         if (zn) tang = (VTI ? c13 : lam) * dX[0] + (VTI ? c13 : lam) * dY[1] + lam * dZ[2];
                         else tang = (VTI ? e2 * l2m - 2.f * c66 : lam) * dY[1] + (VTI ? c13 : lam) * dZ[2] + (VTI ? e2 * lam : lam) * dX[0];
                         const float knn = (VTI && !zn) ? 2.f * e2 * mu : 2.f * mu;
-                        float tn = sg * tang + knn * (zn ? sZ[cn] : sX[cn]) / hn;
-                        if (bc == 2) tn -= ((VTI && !zn) ? 0.5f * sqrtf(e2 * l2m * rho) : 0.5f * sqrtf(l2m * rho)) * pv[cn] / A.dt;
+                        const float ihn = zn ? ihz : ihx;
+                        float tn = sg * tang + knn * (zn ? sZ[cn] : sX[cn]) * ihn;
+                        if (bc == 2) tn -= ((VTI && !zn) ? 0.5f * sqrtf(e2 * l2m * rho) : 0.5f * sqrtf(l2m * rho)) * pv[cn] * idt;
                         acc[cn] -= adh * tn;
                         // tangential components
 #pragma unroll
@@ -774,14 +783,14 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                             if (c == cn) continue;
                             const float G = (VTI && !zn && c == 1) ? c66 : mu;
                             const float dn = c == 0 ? dX[cn] : (c == 1 ? dY[cn] : dZ[cn]);  // D_c of u_cn
-                            float tt = sg * G * dn + G * (zn ? sZ[c] : sX[c]) / hn;
-                            if (bc == 2) tt -= 0.5f * sqrtf(mu * rho) * pv[c] / A.dt;
+                            float tt = sg * G * dn + G * (zn ? sZ[c] : sX[c]) * ihn;
+                            if (bc == 2) tt -= 0.5f * sqrtf(mu * rho) * pv[c] * idt;
                             acc[c] -= adh * tt;
                         }
                         if (bc == 2) {
                             absorb = true;
                             const float F = (VTI && !zn) ? e2 : 1.f;
-                            const float vp = sqrtf(F * l2m / rho), vs = sqrtf(mu / rho);
+                            const float vp = sqrtf(F * l2m * irho), vs = sqrtf(mu * irho);
                             const float k = A.dt / (2.f * hn * SAT_H0);
 #pragma unroll
                             for (int c = 0; c < 3; c++) sc[c] += (c == cn ? vp : vs) * k;
@@ -790,7 +799,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                 }
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    float nv = A.dt2 * acc[c] / rho + 2.f * qc(c, 2) - pv[c];
+                    float nv = A.dt2 * acc[c] * irho + 2.f * qc(c, 2) - pv[c];
                     if (absorb) nv = nv / (1.f + sc[c]);
                     gst(A.next, c * n3 + p, ((nv * tfz) * tfx) * tfy);
                 }
