@@ -715,14 +715,15 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     float dX[3], dY[3], dZ[3], sX[3], sZ[3];
                     {
                         // the 6 values from the x face and from the z face of each component
+                        // (only along the axes whose faces the tile may hold)
                         float ux[3][6], uz[3][6];
 #pragma unroll
                         for (int k = 0; k < 6; k++) {
                             const int xc = xrow(k), zc = zcol(k);
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                ux[c][k] = U(c, xc, lz + 2);
-                                uz[c][k] = U(c, lx + 2, zc);
+                                ux[c][k] = XE ? U(c, xc, lz + 2) : 0.f;
+                                uz[c][k] = ZE ? U(c, lx + 2, zc) : 0.f;
                             }
                         }
                         if (!XE || xr >= 4) {
@@ -758,6 +759,7 @@ __device__ __forceinline__ void elastic_fused_tile(const EArgs& __restrict__ A, 
                     }
                     const int bcz0 = A.bc[0], bcz1 = A.bc[1], bcx0 = A.bc[2], bcx1 = A.bc[3];
                     for (int f = 0; f < 4; f++) {
+                        if (f < 2 ? !ZE : !XE) continue;  // faces the tile cannot hold
                         const int bc = f == 0 ? bcz0 : (f == 1 ? bcz1 : (f == 2 ? bcx0 : bcx1));
                         if (bc != 1 && bc != 2) continue;
                         if (!(f == 0 ? iz == 0 : (f == 1 ? iz == nz - 1 : (f == 2 ? ix == 0 : ix == nx - 1)))) continue;
