@@ -92,8 +92,9 @@ KERNEL_CLASSES = {
 }
 STEP_KERNEL = {
     True: "elastic time step = elastic_fused_batch (middle and edge tiles, planes [6, ny-6)) + elastic_flux_batch "
-          "(the 8 planes at each y face) + elastic_yslabs_ext (y-face slabs, edges, receivers) launches (HIP events "
-          "around each step on the engine stream)",
+          "(the 8 planes at each y face) + elastic_yslabs_ext (y-face slabs, edges, receivers) launches, the y-face "
+          "launches on a second stream beside the edge tiles (HIP events around each step on the engine stream; the "
+          "per-kernel averages come from a run with one stream)",
     False: "elastic time step = elastic_flux + elastic_interior + elastic_boundary + elastic_zslab launches "
            "(HIP events around each step on the engine stream)",
 }

@@ -314,11 +314,13 @@ class Engine:
                                                   C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device_noalloc")
 
-    def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None):
+    def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None, overlap=None):
         """Tiling of the time-step kernels, graph replay of time loops, and the elastic /
         VTI step form (fused, the default: the fused kernel over the planes 6 cells inside
-        the y faces; fchunk: its y planes per workgroup, 0 by grid size) (fwi3d_tune). Results do not
-        depend on the tiling or on graph replay; the two elastic forms agree to rounding."""
+        the y faces; fchunk: its y planes per workgroup, 0 by grid size; overlap: its
+        y-face launches on a second stream, 0 off, 1 from the step's start, 2 from the
+        edge tiles on) (fwi3d_tune). Results do not depend on the tiling, the overlap or
+        graph replay; the two elastic forms agree to rounding."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")
         if tx is not None:
@@ -327,6 +329,8 @@ class Engine:
             self._check(lib().fwi3d_tune(self._h, 3, 1 if fused else 0), "fwi3d_tune")
         if fchunk is not None:
             self._check(lib().fwi3d_tune(self._h, 4, int(fchunk)), "fwi3d_tune")
+        if overlap is not None:
+            self._check(lib().fwi3d_tune(self._h, 5, int(overlap)), "fwi3d_tune")
         if graphs is not None:
             self._check(lib().fwi3d_tune(self._h, 2, int(bool(graphs))), "fwi3d_tune")
 

@@ -79,6 +79,10 @@ struct fwi3d_ctx {
     std::vector<char> shot1_host;
     fwi3d::DevBuf shot1_dev;
     hipStream_t stream = nullptr;
+    // second stream of the fused elastic step (the y-face launches run on it beside the
+    // fused tiles; default priority) and the fork / join events between the two
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // last error message on the context; a server thread and its clients both set it,
     // so it is only touched through fwi3d::set_error / fwi3d_last_error under err_mu
     std::mutex err_mu;
@@ -115,6 +119,10 @@ struct fwi3d_ctx {
     // FWI3D_TUNE_FUSED, default on) and its y chunk (FWI3D_TUNE_FCHUNK, 0: by grid size)
     bool use_fused = true;
     int fused_chunk = 0;
+    // y-face launches of the fused step on the second stream (FWI3D_TUNE_OVERLAP): 0 after
+    // the fused tiles on the same stream, 1 from the start of the step, 2 from the edge
+    // tiles on
+    int fused_overlap = 2;
     fwi3d::DevBuf stepctr;
     std::vector<hipGraphExec_t> graph_execs;  // instantiated in the current call, freed after it
     std::vector<hipGraph_t> graphs;

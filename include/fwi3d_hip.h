@@ -139,14 +139,18 @@ int fwi3d_propagate_device_noalloc(fwi3d_ctx *ctx, int physics, int adj, const f
  * FWI3D_TUNE_FUSED (default 1): elastic / VTI steps update the planes 6 cells inside the
  * y faces with the fused kernel (stresses and fluxes formed on chip; 0: the flux pass,
  * interior, slab and edge kernels over the whole grid). FWI3D_TUNE_FCHUNK: y planes per
- * workgroup of the fused kernel (0: chosen from the grid size). Results agree to
- * rounding between the two elastic forms and depend on none of the other settings. */
+ * workgroup of the fused kernel (0: chosen from the grid size). FWI3D_TUNE_OVERLAP
+ * (default 2): the fused step's y-face launches (their flux pass and slabs) run on a
+ * second stream of the context beside the fused tiles, forked after the middle tiles
+ * (2), at the start of the step (1), or after the tiles on one stream (0). Results agree
+ * to rounding between the two elastic forms and depend on none of the other settings. */
 enum fwi3d_tune_param {
     FWI3D_TUNE_YCHUNK = 0,
     FWI3D_TUNE_TX = 1,
     FWI3D_TUNE_GRAPHS = 2,
     FWI3D_TUNE_FUSED = 3,
-    FWI3D_TUNE_FCHUNK = 4
+    FWI3D_TUNE_FCHUNK = 4,
+    FWI3D_TUNE_OVERLAP = 5
 };
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
 /* Autotuner (no counterpart in the reference): times `steps` time steps of the given
