@@ -212,15 +212,26 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
         (void)threads;
     };
     go(B::C_EXT, extract_batch, 128, dim3(128));
+    // When every elastic / VTI shot runs the fused step, their y-face launches (flux pass of
+    // the 8 planes at each y face, then the y slabs) write points of next no other launch
+    // writes and read no output of one: they run on the context's second stream beside
+    // the fused edge tiles, as in the single-shot step.
+    fwi3d_ctx* c = b->c;
+    bool ovl = c->fused_overlap != 0;
+    for (const auto& sh : b->shots)
+        if (sh.d.phys != FWI3D_ACOUSTIC && !sh.d.fused) ovl = false;
+    auto yflux = [&](hipStream_t sq) {
+        if (b->TX == 8) {
+            go_on(sq, B::C_EFLUX, elastic_flux_batch<8, false>, dim3(64, 8));
+            go_on(sq, B::C_VFLUX, elastic_flux_batch<8, true>, dim3(64, 8));
+        } else {
+            go_on(sq, B::C_EFLUX, elastic_flux_batch<4, false>, dim3(64, 4));
+            go_on(sq, B::C_VFLUX, elastic_flux_batch<4, true>, dim3(64, 4));
+        }
+    };
     // flux passes, then the interiors, shells and z slabs (disjoint points of each shot's
     // next field), then the injections
-    if (b->TX == 8) {
-        go(B::C_EFLUX, elastic_flux_batch<8, false>, 0, dim3(64, 8));
-        go(B::C_VFLUX, elastic_flux_batch<8, true>, 0, dim3(64, 8));
-    } else {
-        go(B::C_EFLUX, elastic_flux_batch<4, false>, 0, dim3(64, 4));
-        go(B::C_VFLUX, elastic_flux_batch<4, true>, 0, dim3(64, 4));
-    }
+    if (!ovl) yflux(st);
     if (b->TX == 8) {
         go(B::C_AINT, acoustic_interior_batch<8>, 0, dim3(64, 8));
         go(B::C_EINT, elastic_interior_batch<8, false>, 0, dim3(64, 8));
@@ -232,6 +243,14 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
     }
     go(B::C_EFUSED, elastic_fused_batch<false, false>, 0, dim3(FTHREADS));
     go(B::C_VFUSED, elastic_fused_batch<true, false>, 0, dim3(FTHREADS));
+    if (ovl) {
+        HIP_OK(hipEventRecord(c->fork_ev, st));
+        HIP_OK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+        yflux(c->side);
+        go_on(c->side, B::C_EYS, elastic_yslabs_batch<false>, dim3(256));
+        go_on(c->side, B::C_VYS, elastic_yslabs_batch<true>, dim3(256));
+        HIP_OK(hipEventRecord(c->join_ev, c->side));
+    }
     go(B::C_EFEDGE, elastic_fused_batch<false, true>, 0, dim3(FTHREADS));
     go(B::C_VFEDGE, elastic_fused_batch<true, true>, 0, dim3(FTHREADS));
     go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
@@ -241,8 +260,12 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
     // shell classes stay empty)
     go(B::C_EZS, elastic_edges_slabs_batch<false>, 0, dim3(256));
     go(B::C_VZS, elastic_edges_slabs_batch<true>, 0, dim3(256));
-    go(B::C_EYS, elastic_yslabs_batch<false>, 0, dim3(256));
-    go(B::C_VYS, elastic_yslabs_batch<true>, 0, dim3(256));
+    if (ovl) {
+        HIP_OK(hipStreamWaitEvent(st, c->join_ev, 0));
+    } else {
+        go(B::C_EYS, elastic_yslabs_batch<false>, 0, dim3(256));
+        go(B::C_VYS, elastic_yslabs_batch<true>, 0, dim3(256));
+    }
     go(B::C_AINJ, acoustic_inject_batch, 0, dim3(256));
     go(B::C_EINJ, elastic_inject_batch<false>, 0, dim3(256));
     go(B::C_VINJ, elastic_inject_batch<true>, 0, dim3(256));
