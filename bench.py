@@ -114,6 +114,44 @@ def traffic_profile(n):
     return os.path.join("profiles", f"elastic_{form}_step_{n}cube_traffic_pmc.md")
 
 
+def kernel_rooflines(fused, nz, nx, ny, avg_us, n):
+    """Per kernel class of the elastic step: the cells it handles, its algorithmic bytes
+    (48 B per cell it updates: u, prev, lambda, mu, rho read, next written; the y-face
+    flux pass reads u, lambda, mu and writes T and 6 fluxes, 48 B per cell too), its
+    one-stream launch time, achieved GB/s and fraction of 8 TB/s, and its measured HBM
+    bytes per launch from the committed PMC summary when there is one."""
+    nxz = nx * nz
+    if fused:
+        cells = {"elastic_fused": nxz * max(0, ny - 12), "elastic_flux_yfaces": nxz * min(ny, 16),
+                 "elastic_yslabs": nxz * min(ny, 12)}
+        prof_keys = {"elastic_fused": ("elastic_fused (middle tiles)", "elastic_fused (edge tiles)"),
+                     "elastic_flux_yfaces": ("elastic_flux_batch (y faces)",), "elastic_yslabs": ("elastic_yslabs_ext",)}
+    else:
+        inner = max(0, nx - 12) * max(0, ny - 12) * max(0, nz - 12)
+        cells = {"elastic_interior": inner, "elastic_flux": nxz * ny, "elastic_boundary": 0,
+                 "elastic_zslab": nxz * ny - inner}
+        prof_keys = {k: (k,) for k in cells}
+    path = os.path.join(ROOT, traffic_profile(n))
+    per_launch = {}
+    if os.path.exists(path):
+        try:
+            block = open(path).read().split("```json", 1)[1].split("```", 1)[0]
+            per_launch = json.loads(block)["traffic_bytes_per_launch"]
+        except (IndexError, KeyError, ValueError):
+            per_launch = {}
+    out = {}
+    for k, c in cells.items():
+        us = avg_us.get(k, 0.0)
+        if c <= 0 or us <= 0:
+            continue
+        b = ELASTIC_BYTES_PER_CELL * c
+        gbs = b / (us * 1e-6) / 1e9
+        t = [per_launch[q] for q in prof_keys[k] if q in per_launch]
+        out[k] = {"cells": int(c), "algorithmic_bytes": int(b), "avg_us": round(us, 2), "achieved_GBs": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic_bytes": int(sum(t)) if t else None}
+    return out
+
+
 def measured_traffic(n):
     """HBM bytes per elastic time step (scaled FETCH_SIZE + WRITE_SIZE of the step kernels)
     from the committed rocprofv3 --pmc summary of the same workload, or None."""
@@ -791,6 +829,7 @@ def main():
     t_pair = avg_us.pop("elastic_time_step") / 1e6
     achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
     traffic = measured_traffic(args.n)
+    per_kernel = kernel_rooflines(not args.unfused, nz, nx, ny, avg_us, args.n)
 
     out = {
         "metric": METRIC,
@@ -831,6 +870,7 @@ def main():
             "bytes_per_cell": ELASTIC_BYTES_PER_CELL,
             "cells_per_step": N,
             "avg_us": {k: round(v, 2) for k, v in avg_us.items()},
+            "per_kernel": per_kernel,
             "launches": {k: n for k, (ms, n) in prof.items() if k != "elastic_time_step"},
             "step_timing": "timed region: HIP events around each time step; per-kernel averages from one untimed shot "
                            "gradient with every kernel bracketed by events",
