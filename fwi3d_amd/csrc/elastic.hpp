@@ -996,24 +996,25 @@ __global__ void elastic_inject(float* __restrict__ out3, EArgs A, InjTab I, cons
 // (we_op.cpp:488-565, 1108-1216) with FP64 accumulation. `lag` is the adjoint field
 // of the previous step (the reference uses the derivatives computed one step earlier);
 // `adj` the current adjoint field (rho term). Gradients in the unpadded layout.
-// Workgroups of 64 (z) x 4 (x) points of one y plane: the forward-field codes and the
+// Workgroups of 64 (z) x GTX (x) points of one y plane: the forward-field codes and the
 // lagged adjoint field of the tile and its 2-cell x / z halos are staged in LDS once, so
 // the interior x and z derivatives read on-chip values (closure rows and the y
 // derivatives read memory, as d1_acc / d1_point do; same arithmetic).
+constexpr int GTX = 4;  // 200^3 event: 472 us (8 x rows: 514, 16: 570; fewer workgroups hide less latency)
 template <bool VTI, int FMT>
-__global__ __launch_bounds__(256) void elastic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> fp,
+__global__ __launch_bounds__(64 * GTX) void elastic_grad(SnapView<FMT> fm, SnapView<FMT> fn, SnapView<FMT> fp,
                                                     const float* __restrict__ adj, const float* __restrict__ lag, EArgs A,
                                                     float sdt, int branch, double* __restrict__ gl, double* __restrict__ gm,
                                                     double* __restrict__ gr) {
-    __shared__ float sf[3][8][68], sl[3][8][68];  // [component][x row x0-2..x0+5][z column z0-2..z0+65]
+    __shared__ float sf[3][GTX + 4][68], sl[3][GTX + 4][68];  // [component][x row from x0-2][z column from z0-2]
     const Geom g = A.g;
     const int lz = threadIdx.x, lx = threadIdx.y;
-    const int z0 = blockIdx.x * 64, x0 = blockIdx.y * 4, iy = blockIdx.z;
+    const int z0 = blockIdx.x * 64, x0 = blockIdx.y * GTX, iy = blockIdx.z;
     const int iz = z0 + lz, ix = x0 + lx;
     const int n3 = g.n;
     const int pl = iy * g.sy;
     const int p = pl + min(ix, g.nx - 1) * g.sx + min(iz, g.nz - 1);
-    HaloMap<4> H;
+    HaloMap<GTX> H;
     H.init(lx, lz, x0, z0, g.nx, g.nz, g.sx);
 #pragma unroll
     for (int c = 0; c < 3; c++) {

@@ -538,16 +538,16 @@ void grad_event_elastic(hipStream_t st, bool vti, const SnapDesc& S, int itl, in
     const int last = nt / sub;
     const int branch = (itl > 0 && itl < last) ? 1 : (itl == 0 ? 0 : 2);
     const Geom& g = A.g;
-    dim3 grid((g.nz + 63) / 64, (g.nx + 3) / 4, g.ny);  // 64 z x 4 x rows per workgroup
+    dim3 grid((g.nz + 63) / 64, (g.nx + GTX - 1) / GTX, g.ny);  // 64 z x GTX x rows per workgroup
     with_snap_fmt(S.fmt, [&](auto tag) {
         constexpr int FMT = decltype(tag)::value;
         const SnapView<FMT> fn = snap_view<FMT>(S, itl), fm = itl > 0 ? snap_view<FMT>(S, itl - 1) : fn,
                             fp = snap_view<FMT>(S, itl + 1);
         if (vti)
-            hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64, 4), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+            hipLaunchKernelGGL((elastic_grad<true, FMT>), grid, dim3(64, GTX), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
                                grad, grad + nunp, grad + 2 * nunp);
         else
-            hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64, 4), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
+            hipLaunchKernelGGL((elastic_grad<false, FMT>), grid, dim3(64, GTX), 0, st, fm, fn, fp, adjf, lagf, A, sdt, branch,
                                grad, grad + nunp, grad + 2 * nunp);
     });
 }
