@@ -314,12 +314,13 @@ class Engine:
                                                   C.c_float(d[1]), C.c_float(d[2]))
         self._check(rc, "fwi3d_propagate_device_noalloc")
 
-    def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None, overlap=None):
+    def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None, overlap=None, echunk=None):
         """Tiling of the time-step kernels, graph replay of time loops, and the elastic /
         VTI step form (fused, the default: the fused kernel over the planes 6 cells inside
         the y faces; fchunk: its y planes per workgroup, 0 by grid size; overlap: its
         y-face launches on a second stream, 0 off, 1 from the step's start, 2 from the
-        edge tiles on) (fwi3d_tune). Results do not depend on the tiling, the overlap or
+        edge tiles on; echunk: y planes per workgroup of its tiles at the x or z faces, 0
+        as fchunk) (fwi3d_tune). Results do not depend on the tiling, the overlap or
         graph replay; the two elastic forms agree to rounding."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")
@@ -331,6 +332,8 @@ class Engine:
             self._check(lib().fwi3d_tune(self._h, 4, int(fchunk)), "fwi3d_tune")
         if overlap is not None:
             self._check(lib().fwi3d_tune(self._h, 5, int(overlap)), "fwi3d_tune")
+        if echunk is not None:
+            self._check(lib().fwi3d_tune(self._h, 6, int(echunk)), "fwi3d_tune")
         if graphs is not None:
             self._check(lib().fwi3d_tune(self._h, 2, int(bool(graphs))), "fwi3d_tune")
 

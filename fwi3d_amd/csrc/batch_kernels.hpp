@@ -196,12 +196,12 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(VTI ? 
     const int l = L - m.first(s);
     if (EDGE) {
         // the side tiles first, then the corner tiles in their shorter y chunks
-        const int per = fused_side_tiles(FT.ntz, FT.ntx), ns = per * FT.nyc;
+        const int per = fused_side_tiles(FT.ntz, FT.ntx), ns = per * FT.snyc;
         const int ncor = fused_corner_tiles(FT.ntz, FT.ntx);
         const bool side = l < ns;
         const int e = side ? l : l - ns, q = side ? per : ncor;
         const int tzx = side ? fused_side_tile(e % q, FT.ntz, FT.ntx) : fused_corner_tile(e % q, FT.ntz, FT.ntx);
-        const int yc = side ? FT.ychunk : FT.cychunk, ya = FT.ya + (e / q) * yc;
+        const int yc = side ? FT.sychunk : FT.cychunk, ya = FT.ya + (e / q) * yc;
         elastic_fused_tile<VTI, true, true>(A, FT, tzx & 0xffff, tzx >> 16, ya, min(FT.yb, ya + yc), smem);
     } else {
         const int mz = FT.ntz - 2, per = mz * (FT.ntx - 2);

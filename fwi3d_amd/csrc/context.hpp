@@ -110,6 +110,7 @@ struct fwi3d_ctx {
     // tiles chosen by fwi3d_autotune per physics and grid (they take precedence there)
     struct Tiles {
         int physics, nx, ny, nz, tx, ychunk;
+        int echunk;  // side-tile y chunk of the fused elastic / VTI kernel (0: by grid size)
     };
     std::vector<Tiles> tuned;
     // time loops without host work between steps replay a captured graph of several steps
@@ -119,6 +120,7 @@ struct fwi3d_ctx {
     // FWI3D_TUNE_FUSED, default on) and its y chunk (FWI3D_TUNE_FCHUNK, 0: by grid size)
     bool use_fused = true;
     int fused_chunk = 0;
+    int fused_echunk = 0;  // y chunk of its side tiles (FWI3D_TUNE_ECHUNK, 0: fused_chunk's)
     // y-face launches of the fused step on the second stream (FWI3D_TUNE_OVERLAP): 0 after
     // the fused tiles on the same stream, 1 from the start of the step, 2 from the edge
     // tiles on

@@ -379,7 +379,15 @@ int shell_edge_count(const Geom& g) { return edge_count(6, g.nx, g.ny, g.nz); }
 struct FusedPlan {
     int nfused, nfedge, nflux, nslab;  // fused middle / edge workgroups, flux pass, slab launch
 };
-FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk) {
+// Side-tile y chunk giving about `target` side workgroups (at least 8 planes: each chunk
+// spends 4 iterations filling its queues).
+int fused_side_chunk(int ntz, int ntx, int nyo, int target) {
+    const int sides = std::max(1, fused_side_tiles(ntz, ntx));
+    const int want = std::max(1, (target + sides / 2) / sides);
+    return std::max(8, (nyo + want - 1) / want);
+}
+
+FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk, int echunk) {
     FusedPlan P{};
     d.fused = 1;
     FusedTiles& F = d.ft;
@@ -399,11 +407,18 @@ FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk) {
     }
     F.ychunk = std::max(1, std::min(yc, std::max(nyo, 1)));
     F.nyc = nyo > 0 ? (nyo + F.ychunk - 1) / F.ychunk : 0;
-    // corner tiles: y chunks of a quarter of the others' (at least 4 planes)
-    F.cychunk = std::max(1, std::min(std::max(4, (F.ychunk + 3) / 4), std::max(nyo, 1)));
+    // Edge tiles run one 512-thread workgroup per CU. Side tiles in y chunks of `echunk`
+    // planes (0: about 384 side workgroups, 1.5 per CU of the 256); corner tiles (the
+    // costliest per plane) in chunks of half that length, at least 4 planes. Measured
+    // (edge launch): 200^3 230 against 265 us with the middle tiles' chunk (21 planes) for
+    // the side tiles and a quarter of it for the corners; 256^3 295 against 400 us.
+    const int sc = echunk > 0 ? echunk : fused_side_chunk(F.ntz, F.ntx, nyo, 384);
+    F.sychunk = std::max(1, std::min(sc, std::max(nyo, 1)));
+    F.snyc = nyo > 0 ? (nyo + F.sychunk - 1) / F.sychunk : 0;
+    F.cychunk = std::max(1, std::min(std::max(4, F.sychunk / 2), std::max(nyo, 1)));
     F.cnyc = nyo > 0 ? (nyo + F.cychunk - 1) / F.cychunk : 0;
     P.nfused = fused_middle_tiles(F.ntz, F.ntx) * F.nyc;
-    P.nfedge = fused_corner_tiles(F.ntz, F.ntx) * F.cnyc + fused_side_tiles(F.ntz, F.ntx) * F.nyc;
+    P.nfedge = fused_corner_tiles(F.ntz, F.ntx) * F.cnyc + fused_side_tiles(F.ntz, F.ntx) * F.snyc;
     EArgs& A = d.ef;
     if (g.ny >= 16) {
         A.org = 0;
@@ -426,14 +441,17 @@ FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk) {
 }
 
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
-// grid on the context, else the context's settings (fwi3d_tune).
-void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk) {
+// grid on the context, else the context's settings (fwi3d_tune). `echunk`: y chunk of
+// the fused kernel's side tiles.
+void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk, int& echunk) {
     TX = c->tune_tx;
     ychunk = c->tune_ychunk;
+    echunk = c->fused_echunk;
     for (const auto& t : c->tuned)
         if (t.physics == physics && t.nx == g.nx && t.ny == g.ny && t.nz == g.nz) {
             TX = t.tx;
             ychunk = t.ychunk;
+            if (t.echunk > 0) echunk = t.echunk;
         }
 }
 
@@ -578,8 +596,8 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     float* curr = F + n;
     float* next = F + 2 * n;
 
-    int TX, ychunk;
-    tiles_for(c, FWI3D_ACOUSTIC, g, TX, ychunk);
+    int TX, ychunk, echunk;
+    tiles_for(c, FWI3D_ACOUSTIC, g, TX, ychunk, echunk);
     AcuArgs A{};
     A.K = M; A.b = M + n; A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2; A.g = g;
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
@@ -799,8 +817,8 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
     inject(prev, 0, 1);
 
-    int TX, ychunk;
-    tiles_for(c, vti ? FWI3D_VTI : FWI3D_ELASTIC, g, TX, ychunk);
+    int TX, ychunk, echunk;
+    tiles_for(c, vti ? FWI3D_VTI : FWI3D_ELASTIC, g, TX, ychunk, echunk);
     const int nblk = tile_grid_yc(A, g, TX, false, ychunk);
     EArgs AI = A;
     const int nint = tile_grid_yc(AI, g, TX, true, ychunk);
@@ -813,7 +831,7 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int nzslab = zslab_tiles(T1h.zs, g);
     const bool fused = c->use_fused;
     FusedPlan FP{};
-    if (fused) FP = fused_plan(T1h, g, TX, ychunk, c->fused_chunk);
+    if (fused) FP = fused_plan(T1h, g, TX, ychunk, c->fused_chunk, echunk);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     T1h.E = E; T1h.rcv = d_rcv; T1h.next_blk = (E.npts + 127) / 128;
     const BShot* T1 = upload_shot1(c, T1h);
@@ -1234,6 +1252,11 @@ int fwi3d_tune(fwi3d_ctx* c, int param, int value) {
         c->fused_chunk = value;
         return FWI3D_OK;
     }
+    if (param == FWI3D_TUNE_ECHUNK) {
+        if (value < 0) return fail(c, Err{FWI3D_EINVAL, "edge y chunk must be >= 0 (0: the fused y chunk)"});
+        c->fused_echunk = value;
+        return FWI3D_OK;
+    }
     if (param == FWI3D_TUNE_OVERLAP) {
         if (value < 0 || value > 2) return fail(c, Err{FWI3D_EINVAL, "overlap must be 0, 1 or 2"});
         c->fused_overlap = value;
@@ -1249,9 +1272,9 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
     long long n0[fwi3d_ctx::kProfClasses];
     std::copy(c->prof_ms, c->prof_ms + fwi3d_ctx::kProfClasses, ms0);
     std::copy(c->prof_n, c->prof_n + fwi3d_ctx::kProfClasses, n0);
-    const int tx0 = c->tune_tx, yc0 = c->tune_ychunk;
+    const int tx0 = c->tune_tx, yc0 = c->tune_ychunk, ec0 = c->fused_echunk;
     auto restore = [&] {
-        c->tune_tx = tx0; c->tune_ychunk = yc0; c->prof = prof0; c->prof_steps_only = steps_only0;
+        c->tune_tx = tx0; c->tune_ychunk = yc0; c->fused_echunk = ec0; c->prof = prof0; c->prof_steps_only = steps_only0;
         std::copy(ms0, ms0 + fwi3d_ctx::kProfClasses, c->prof_ms);
         std::copy(n0, n0 + fwi3d_ctx::kProfClasses, c->prof_n);
     };
@@ -1304,15 +1327,29 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             return c->prof_ms[4] / steps;  // whole time steps (the kernels overlap)
         };
         run(4, 8);  // warm-up
-        int best_tx = 4, best_yc = 8;
+        int best_tx = 4, best_yc = 8, best_ec = 0;
         double best = 1e30;
         for (int tx : {4, 8})
             for (int yc : {4, 8, 16, 32}) {
                 const double t = run(tx, yc);
                 if (t < best) { best = t; best_tx = tx; best_yc = yc; }
             }
+        if (physics != FWI3D_ACOUSTIC && c->use_fused && ny > 12) {
+            // y chunk of the fused kernel's side tiles: about 1, 1.5, 2 or 2.5 side
+            // workgroups per CU (the edge tiles run one workgroup per CU)
+            const int ntz = fused_tile_count(nz, FZ), ntx = fused_tile_count(nx, FW), nyo = ny - 12;
+            int tried[4], ntried = 0;
+            for (int target : {256, 384, 512, 640}) {
+                const int ec = fused_side_chunk(ntz, ntx, nyo, target);
+                if (std::find(tried, tried + ntried, ec) != tried + ntried) continue;
+                tried[ntried++] = ec;
+                c->fused_echunk = ec;
+                const double t = run(best_tx, best_yc);
+                if (t < best) { best = t; best_ec = ec; }
+            }
+        }
         restore();
-        c->tuned.push_back(fwi3d_ctx::Tiles{physics, nx, ny, nz, best_tx, best_yc});
+        c->tuned.push_back(fwi3d_ctx::Tiles{physics, nx, ny, nz, best_tx, best_yc, best_ec});
         if (tx_out) *tx_out = best_tx;
         if (ychunk_out) *ychunk_out = best_yc;
         m.release();

@@ -36,6 +36,7 @@ struct FusedTiles {
     int ntz, ntx, nyc, ychunk;
     int ya, yb;          // output planes [ya, yb) = [6, ny-6)
     int cnyc, cychunk;   // y chunks of the corner tiles (shorter: they are the costliest)
+    int snyc, sychunk;   // y chunks of the side tiles (edge tiles with closures along one axis)
 };
 
 __host__ __device__ inline int fused_tile_count(int n, int w) {

@@ -143,19 +143,24 @@ int fwi3d_propagate_device_noalloc(fwi3d_ctx *ctx, int physics, int adj, const f
  * (default 2): the fused step's y-face launches (their flux pass and slabs) run on a
  * second stream of the context beside the fused tiles, forked after the middle tiles
  * (2), at the start of the step (1), or after the tiles on one stream (0). Results agree
- * to rounding between the two elastic forms and depend on none of the other settings. */
+ * to rounding between the two elastic forms and depend on none of the other settings.
+ * FWI3D_TUNE_ECHUNK: y planes per workgroup of the fused kernel's side tiles (the tiles
+ * at the x or z faces; 0: the middle tiles' chunk). */
 enum fwi3d_tune_param {
     FWI3D_TUNE_YCHUNK = 0,
     FWI3D_TUNE_TX = 1,
     FWI3D_TUNE_GRAPHS = 2,
     FWI3D_TUNE_FUSED = 3,
     FWI3D_TUNE_FCHUNK = 4,
-    FWI3D_TUNE_OVERLAP = 5
+    FWI3D_TUNE_OVERLAP = 5,
+    FWI3D_TUNE_ECHUNK = 6
 };
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
 /* Autotuner (no counterpart in the reference): times `steps` time steps of the given
  * physics and grid on a synthetic model for every tile shape (TX 4 or 8, y chunk 4, 8,
- * 16 or 32) with HIP events, keeps the fastest for that physics and grid on the context
+ * 16 or 32, and for the fused elastic / VTI step the y chunk of its side tiles: about 1,
+ * 1.5, 2 or 2.5 side workgroups per CU) with HIP events, keeps the fastest for that
+ * physics and grid on the context
  * (used by propagate and by batch admission in preference to fwi3d_tune's settings) and
  * returns it. */
 int fwi3d_autotune(fwi3d_ctx *ctx, int physics, int nx, int ny, int nz, int steps, int *tx, int *ychunk);

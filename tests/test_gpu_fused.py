@@ -147,3 +147,30 @@ def test_overlap_modes_identical(eng, physics):
     assert np.isfinite(ref).all() and np.abs(ref).max() > 0
     for k, v in out.items():
         assert np.array_equal(v, ref), k
+
+
+@pytest.mark.parametrize("physics", ["elastic", "vti"])
+def test_edge_chunks_identical(eng, physics):
+    """The y chunk of the fused kernel's side tiles (fwi3d_tune FWI3D_TUNE_ECHUNK; the
+    corner tiles take half of it) only changes which workgroup updates a plane: traces
+    are bitwise equal for every chunk, including one past the planes' count and the
+    autotuner's choice."""
+    shape, nt = (40, 50, 70), 40
+    print(f"[edge chunks {physics}] shape={shape} nt={nt}", flush=True)
+    model, src, sloc, rloc, d, dt = small_shot(physics, 1, shape=shape, nt=nt)
+    inj, ext = F.Injector(1, sloc, shape, d), F.Injector(0, rloc, shape, d)
+    par = F.make_param(nt, dt, bc=CASES["mixed"]["bc"], taper=CASES["mixed"]["taper"])
+    out = {}
+    try:
+        for ec in (0, 1, 5, 8, 13, 100):
+            eng.tune(echunk=ec)
+            out[ec] = eng.propagate(physics, model, src, inj, ext, par, d)
+        eng.tune(echunk=0)
+        eng.autotune(physics, shape, steps=2)
+        out["auto"] = eng.propagate(physics, model, src, inj, ext, par, d)
+    finally:
+        eng.tune(echunk=0)
+    ref = out[0]
+    assert np.isfinite(ref).all() and np.abs(ref).max() > 0
+    for k, v in out.items():
+        assert np.array_equal(v, ref), k
