@@ -410,8 +410,9 @@ FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk, in
     // Edge tiles run one 512-thread workgroup per CU. Side tiles in y chunks of `echunk`
     // planes (0: about 384 side workgroups, 1.5 per CU of the 256); corner tiles (the
     // costliest per plane) in chunks of half that length, at least 4 planes. Measured
-    // (edge launch): 200^3 230 against 265 us with the middle tiles' chunk (21 planes) for
-    // the side tiles and a quarter of it for the corners; 256^3 295 against 400 us.
+    // (middle + edge launches): 200^3 230 against 265 us with the middle tiles' chunk
+    // (21 planes) for the side tiles and a quarter of it for the corners; 256^3 363
+    // against 468 us. Grids where other chunks win are left to fwi3d_autotune.
     const int sc = echunk > 0 ? echunk : fused_side_chunk(F.ntz, F.ntx, nyo, 384);
     F.sychunk = std::max(1, std::min(sc, std::max(nyo, 1)));
     F.snyc = nyo > 0 ? (nyo + F.sychunk - 1) / F.sychunk : 0;
