@@ -82,7 +82,7 @@ def cpu_baseline(shot, seconds=6.0, gradient=True):
                       f"(oracle FP32 C++/OpenMP); median of 3 runs after one warm-up"}
 
 
-# fwi3d_tune settings of every engine the workloads create (--unfused)
+# fwi3d_tune settings of every engine the workloads create (--unfused, --overlap)
 ENGINE_TUNE = {}
 
 # profiled kernel classes of the elastic time step (fwi3d_profile_read), by step form
@@ -92,9 +92,9 @@ KERNEL_CLASSES = {
 }
 STEP_KERNEL = {
     True: "elastic time step = elastic_fused_batch (middle and edge tiles, planes [6, ny-6)) + elastic_flux_batch "
-          "(the 8 planes at each y face) + elastic_yslabs_ext (y-face slabs, edges, receivers) launches, the y-face "
-          "launches on a second stream beside the edge tiles (HIP events around each step on the engine stream; the "
-          "per-kernel averages come from a run with one stream)",
+          "(the 8 planes at each y face) + elastic_yslabs_ext (y-face slabs, edges, receivers) launches, the edge "
+          "tiles on a second stream beside the middle tiles and the y-face launches after the middle tiles (HIP "
+          "events around each step on the engine stream; the per-kernel averages come from a run with one stream)",
     False: "elastic time step = elastic_flux + elastic_interior + elastic_boundary + elastic_zslab launches "
            "(HIP events around each step on the engine stream)",
 }
@@ -716,6 +716,8 @@ def main():
     ap.add_argument("--unfused", action="store_true",
                     help="elastic / VTI steps with the flux, interior, slab and edge kernels (fwi3d_tune "
                          "FWI3D_TUNE_FUSED = 0) instead of the fused kernel")
+    ap.add_argument("--overlap", type=int, default=None, choices=(0, 1, 2, 3),
+                    help="streams of the fused step (fwi3d_tune FWI3D_TUNE_OVERLAP; default: the library's, 3)")
     ap.add_argument("--workload", default="gradient", choices=("gradient", "acoustic", "lbfgs", "batch", "serve"),
                     help="gradient: elastic shot gradient (default); acoustic: 100^3 acoustic forward modelling; "
                          "lbfgs: 256^3 mixed-precision l-BFGS FWI; batch: 32 VTI shots on 3 grids, batched vs one at "
@@ -729,6 +731,8 @@ def main():
     args = ap.parse_args()
     if args.unfused:
         ENGINE_TUNE["fused"] = False
+    if args.overlap is not None:
+        ENGINE_TUNE["overlap"] = args.overlap
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (shots would shard one per GPU); "
               "use --gpus 1", file=sys.stderr)

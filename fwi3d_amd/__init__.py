@@ -317,10 +317,10 @@ class Engine:
     def tune(self, ychunk=None, tx=None, graphs=None, fused=None, fchunk=None, overlap=None, echunk=None):
         """Tiling of the time-step kernels, graph replay of time loops, and the elastic /
         VTI step form (fused, the default: the fused kernel over the planes 6 cells inside
-        the y faces; fchunk: its y planes per workgroup, 0 by grid size; overlap: its
-        y-face launches on a second stream, 0 off, 1 from the step's start, 2 from the
-        edge tiles on; echunk: y planes per workgroup of its tiles at the x or z faces, 0
-        as fchunk) (fwi3d_tune). Results do not depend on the tiling, the overlap or
+        the y faces; fchunk: its y planes per workgroup, 0 by grid size; overlap: 0 one
+        stream, 1 / 2 its y-face launches on a second stream from the step's start / from
+        the edge tiles on, 3 its edge tiles on a second stream beside the middle tiles;
+        echunk: y planes per workgroup of its side tiles, 0 by grid size) (fwi3d_tune). Results do not depend on the tiling, the overlap or
         graph replay; the two elastic forms agree to rounding."""
         if ychunk is not None:
             self._check(lib().fwi3d_tune(self._h, 0, int(ychunk)), "fwi3d_tune")

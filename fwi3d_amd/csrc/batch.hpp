@@ -212,10 +212,11 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
         (void)threads;
     };
     go(B::C_EXT, extract_batch, 128, dim3(128));
-    // When every elastic / VTI shot runs the fused step, their y-face launches (flux pass of
-    // the 8 planes at each y face, then the y slabs) write points of next no other launch
-    // writes and read no output of one: they run on the context's second stream beside
-    // the fused edge tiles, as in the single-shot step.
+    // When every elastic / VTI shot runs the fused step, its launches write disjoint points
+    // of next and read no output of each other: as in the single-shot step, the fused edge
+    // tiles run on the context's second stream beside the middle tiles (FWI3D_TUNE_OVERLAP
+    // 3), or the y-face launches (flux pass of the 8 planes at each y face, then the y
+    // slabs) beside the edge tiles (1, 2).
     fwi3d_ctx* c = b->c;
     bool ovl = c->fused_overlap != 0;
     for (const auto& sh : b->shots)
@@ -229,9 +230,17 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
             go_on(sq, B::C_VFLUX, elastic_flux_batch<4, true>, dim3(64, 4));
         }
     };
+    const bool edge_side = ovl && c->fused_overlap == 3;
     // flux passes, then the interiors, shells and z slabs (disjoint points of each shot's
     // next field), then the injections
     if (!ovl) yflux(st);
+    if (edge_side) {
+        HIP_OK(hipEventRecord(c->fork_ev, st));
+        HIP_OK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+        go_on(c->side, B::C_EFEDGE, elastic_fused_batch<false, true>, dim3(FTHREADS));
+        go_on(c->side, B::C_VFEDGE, elastic_fused_batch<true, true>, dim3(FTHREADS));
+        HIP_OK(hipEventRecord(c->join_ev, c->side));
+    }
     if (b->TX == 8) {
         go(B::C_AINT, acoustic_interior_batch<8>, 0, dim3(64, 8));
         go(B::C_EINT, elastic_interior_batch<8, false>, 0, dim3(64, 8));
@@ -243,16 +252,22 @@ void batch_one_step(fwi3d_batch* b, StepIdx it) {
     }
     go(B::C_EFUSED, elastic_fused_batch<false, false>, 0, dim3(FTHREADS));
     go(B::C_VFUSED, elastic_fused_batch<true, false>, 0, dim3(FTHREADS));
-    if (ovl) {
-        HIP_OK(hipEventRecord(c->fork_ev, st));
-        HIP_OK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
-        yflux(c->side);
-        go_on(c->side, B::C_EYS, elastic_yslabs_batch<false>, dim3(256));
-        go_on(c->side, B::C_VYS, elastic_yslabs_batch<true>, dim3(256));
-        HIP_OK(hipEventRecord(c->join_ev, c->side));
+    if (edge_side) {
+        yflux(st);
+        go(B::C_EYS, elastic_yslabs_batch<false>, 0, dim3(256));
+        go(B::C_VYS, elastic_yslabs_batch<true>, 0, dim3(256));
+    } else {
+        if (ovl) {
+            HIP_OK(hipEventRecord(c->fork_ev, st));
+            HIP_OK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+            yflux(c->side);
+            go_on(c->side, B::C_EYS, elastic_yslabs_batch<false>, dim3(256));
+            go_on(c->side, B::C_VYS, elastic_yslabs_batch<true>, dim3(256));
+            HIP_OK(hipEventRecord(c->join_ev, c->side));
+        }
+        go(B::C_EFEDGE, elastic_fused_batch<false, true>, 0, dim3(FTHREADS));
+        go(B::C_VFEDGE, elastic_fused_batch<true, true>, 0, dim3(FTHREADS));
     }
-    go(B::C_EFEDGE, elastic_fused_batch<false, true>, 0, dim3(FTHREADS));
-    go(B::C_VFEDGE, elastic_fused_batch<true, true>, 0, dim3(FTHREADS));
     go(B::C_ASHELL, acoustic_boundary_batch, 0, dim3(256));
     go(B::C_ESHELL, elastic_boundary_batch<false>, 0, dim3(256));
     go(B::C_VSHELL, elastic_boundary_batch<true>, 0, dim3(256));

@@ -121,10 +121,10 @@ struct fwi3d_ctx {
     bool use_fused = true;
     int fused_chunk = 0;
     int fused_echunk = 0;  // y chunk of its side tiles (FWI3D_TUNE_ECHUNK, 0: fused_chunk's)
-    // y-face launches of the fused step on the second stream (FWI3D_TUNE_OVERLAP): 0 after
-    // the fused tiles on the same stream, 1 from the start of the step, 2 from the edge
-    // tiles on
-    int fused_overlap = 2;
+    // launches of the fused step on the second stream (FWI3D_TUNE_OVERLAP): 0 none, 1 the
+    // y faces from the start of the step, 2 the y faces from the edge tiles on, 3 the edge
+    // tiles from the start of the step
+    int fused_overlap = 3;
     fwi3d::DevBuf stepctr;
     std::vector<hipGraphExec_t> graph_execs;  // instantiated in the current call, freed after it
     std::vector<hipGraph_t> graphs;

@@ -140,9 +140,10 @@ int fwi3d_propagate_device_noalloc(fwi3d_ctx *ctx, int physics, int adj, const f
  * y faces with the fused kernel (stresses and fluxes formed on chip; 0: the flux pass,
  * interior, slab and edge kernels over the whole grid). FWI3D_TUNE_FCHUNK: y planes per
  * workgroup of the fused kernel (0: chosen from the grid size). FWI3D_TUNE_OVERLAP
- * (default 2): the fused step's y-face launches (their flux pass and slabs) run on a
- * second stream of the context beside the fused tiles, forked after the middle tiles
- * (2), at the start of the step (1), or after the tiles on one stream (0). Results agree
+ * (default 3): the fused step's edge tiles run on a second stream of the context beside
+ * its middle tiles, the y-face launches (their flux pass and slabs) after the middle
+ * tiles (3); or the y-face launches run on the second stream, forked after the middle
+ * tiles (2) or at the start of the step (1); or all on one stream (0). Results agree
  * to rounding between the two elastic forms and depend on none of the other settings.
  * FWI3D_TUNE_ECHUNK: y planes per workgroup of the fused kernel's side tiles (the tiles
  * at the x or z faces; 0: the middle tiles' chunk). */

@@ -127,9 +127,10 @@ def test_fused_equals_unfused(eng, physics):
 
 @pytest.mark.parametrize("physics", ["elastic", "vti"])
 def test_overlap_modes_identical(eng, physics):
-    """The fused step's y-face launches on the second stream (fwi3d_tune
-    FWI3D_TUNE_OVERLAP 1, 2) or on one stream (0), with and without graph replay: the
-    launches write disjoint points of the next field, so the traces are bitwise equal."""
+    """The fused step's edge tiles (fwi3d_tune FWI3D_TUNE_OVERLAP 3) or y-face launches
+    (1, 2) on the second stream, or all on one stream (0), with and without graph replay:
+    the launches write disjoint points of the next field, so the traces are bitwise
+    equal."""
     shape, nt = (40, 50, 70), 50
     print(f"[overlap modes {physics}] shape={shape} nt={nt}", flush=True)
     model, src, sloc, rloc, d, dt = small_shot(physics, 1, shape=shape, nt=nt)
@@ -137,12 +138,12 @@ def test_overlap_modes_identical(eng, physics):
     par = F.make_param(nt, dt, bc=CASES["mixed"]["bc"], taper=CASES["mixed"]["taper"])
     out = {}
     try:
-        for ov in (0, 1, 2):
+        for ov in (0, 1, 2, 3):
             for graphs in (True, False):
                 eng.tune(overlap=ov, graphs=graphs)
                 out[ov, graphs] = eng.propagate(physics, model, src, inj, ext, par, d)
     finally:
-        eng.tune(overlap=2, graphs=True)
+        eng.tune(overlap=3, graphs=True)
     ref = out[0, False]
     assert np.isfinite(ref).all() and np.abs(ref).max() > 0
     for k, v in out.items():
