@@ -827,8 +827,8 @@ def main():
 
     updates = 2.0 * N * (nt - 1) * args.steps  # forward + adjoint cell-updates
     gcell = updates / elapsed / 1e9
-    # one elastic time step = flux pre-pass + interior y-march + boundary shell + z-slab
-    # launches, timed as a whole (HIP events around the step on the engine stream)
+    # one elastic time step = the launches of its step form (STEP_KERNEL), timed as a whole
+    # (HIP events around the step on the engine stream, after both streams join)
     avg_us = {k: ms / max(n, 1) * 1e3 for k, (ms, n) in prof.items()}
     t_pair = avg_us.pop("elastic_time_step") / 1e6
     achieved = ELASTIC_BYTES_PER_CELL * N / t_pair / 1e9 if t_pair > 0 else 0.0
