@@ -311,6 +311,8 @@ def run_batch(args, torch, F, W, emit=True):
     for s, inj, ext, par, m, src, _, r2 in bufs:
         b.add("vti", m.data_ptr(), src.data_ptr(), r2.data_ptr(), inj, ext, par, s.shape, s.d)
     resident = b.status()[1]
+    torch.cuda.synchronize()
+    t_admit = time.perf_counter() - t0
     b.autotune(window=10)  # online: the first 40 steps time y chunks 4/8/16/32
     b.run()
     batch_tiles = b.tiles()
@@ -342,6 +344,7 @@ def run_batch(args, torch, F, W, emit=True):
         "plain_latency_s": {"p50": round(float(np.percentile(lat_plain, 50)), 4),
                             "p99": round(float(np.percentile(lat_plain, 99)), 4)},
         "batch_resident_gib": round(resident / 2**30, 2),
+        "batch_phase_s": {"admit": round(t_admit, 3), "steps": round(t_batch - t_admit, 3)},
         "autotuned_tiles_tx_ychunk": tiles,
         "batch_online_ychunk": batch_tiles[0],
         "batch_candidate_ms_per_step": {str(y): round(t, 3) for y, t in zip((4, 8, 16, 32), batch_tiles[1])},
