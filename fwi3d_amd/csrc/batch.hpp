@@ -612,7 +612,7 @@ long long fwi3d_batch_snap_footprint(int physics, int nx, int ny, int nz, int nt
     }
 }
 
-// Batches whose fused elastic / VTI steps run on two streams and that hold 4 Mi cells or
+// Batches whose fused elastic / VTI steps run on two streams and that hold 3 Mi cells or
 // more launch their steps directly, as single shots of that size do: replayed from a
 // captured graph, the two-stream step of the 32 VTI shots of the batched configuration
 // took 7.01 against 6.44 ms launched directly.
@@ -625,7 +625,7 @@ bool batch_launch_directly(const fwi3d_batch* b) {
         fused = fused || sh.d.fused;
         cells += (long long)sh.d.g.nx * sh.d.g.ny * sh.d.g.nz;
     }
-    return fused && cells >= (4LL << 20);
+    return fused && cells >= (3LL << 20);
 }
 
 int fwi3d_batch_step(fwi3d_batch* b, int nsteps, int* nactive) {

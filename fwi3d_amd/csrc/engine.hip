@@ -889,10 +889,12 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
 
     if (!fwd_snap && !(adj && d_grad && sub > 0)) {
-        // no host work between steps: graph replay, except for large fused steps whose
-        // y-face launches run on the second stream (replayed from a graph, the two-stream
-        // step measured 315 against 300 us launched directly at 200^3)
-        if (fused && c->fused_overlap != 0 && (long long)g.nx * g.ny * g.nz >= (4LL << 20))
+        // no host work between steps: graph replay, except for fused two-stream steps of
+        // 3 Mi cells or more, launched directly. Replayed from a graph against launched
+        // directly, per step: 200^3 elastic 256 / 258 us (y faces on the second stream:
+        // 288 / 271), VTI 96 x 192 x 192 (nz, nx, ny) 236 / 214 us; below the threshold
+        // VTI 128^3 140 / 146 us, elastic 128^3 113 / 121 us.
+        if (fused && c->fused_overlap != 0 && (long long)g.nx * g.ny * g.nz >= (3LL << 20))
             for (int it = 0; it < nt - 1; it++) step(StepIdx(it));
         else
             run_steps(c, 0, nt - 1, step);
