@@ -1353,7 +1353,9 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             }
         if (physics != FWI3D_ACOUSTIC && c->use_fused && ny > 12) {
             // y chunk of the fused kernel's side tiles: about 1, 1.5, 2 or 2.5 side
-            // workgroups per CU (the edge tiles run one workgroup per CU)
+            // workgroups per CU (the edge tiles run one workgroup per CU). A chunk other
+            // than the grid-size rule's must be 3 % faster: the timings of 10-step runs
+            // vary by a few per cent, and a noisy pick cost a batch of three grids 10 %.
             const int ntz = fused_tile_count(nz, FZ), ntx = fused_tile_count(nx, FW), nyo = ny - 12;
             int tried[4], ntried = 0;
             for (int target : {256, 384, 512, 640}) {
@@ -1362,7 +1364,7 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
                 tried[ntried++] = ec;
                 c->fused_echunk = ec;
                 const double t = run(best_tx, best_yc);
-                if (t < best) { best = t; best_ec = ec; }
+                if (t < 0.97 * best) { best = t; best_ec = ec; }
             }
         }
         restore();
