@@ -672,6 +672,7 @@ def config_d_gradients(torch, F, W, n=256, nt=3000):
     shape = true.shape
     dev = torch.device("cuda:0")
     eng = new_engine(F)
+    tiles = eng.autotune("elastic", shape, steps=10)  # as the headline workload does
     geo = F.Shot(shape, true.d, (0.0, 0.0, 0.0), nt, true.dt, true.src_loc[0], true.rcv_loc)
     opkw = dict(nmodels=3, dt=true.dt, sub=true.sub, mt=1, bc=true.bc, taper=true.taper,
                 taper_strength=true.taper_strength)
@@ -701,7 +702,8 @@ def config_d_gradients(torch, F, W, n=256, nt=3000):
     eng.close()
     return {"workload": f"elastic shot gradient {shape[2]}x{shape[1]}x{shape[0]}, nt={nt}, sub={true.sub}, FP32 and "
                         f"BF16 (per-snapshot max-abs scale) snapshots, FP64 gradient accumulation",
-            **res, "bf16_vs_f32_gradient_rel_l2": {"vp": err[0], "vs": err[1], "rho": err[2]}}
+            **res, "bf16_vs_f32_gradient_rel_l2": {"vp": err[0], "vs": err[1], "rho": err[2]},
+            "autotuned_tiles_tx_ychunk": list(tiles)}
 
 
 def main():

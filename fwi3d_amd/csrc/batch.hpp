@@ -446,8 +446,8 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
         d.src = d_src;
         d.rcv = d_rcv;
         const float dt = par->dt;
-        int tx_tuned, yc, echunk;  // the batch's launches share one TX; the y chunks are per shot
-        tiles_for(c, physics, g, tx_tuned, yc, echunk);
+        int tx_tuned, yc, echunk, fchunk;  // the batch's launches share one TX; the y chunks are per shot
+        tiles_for(c, physics, g, tx_tuned, yc, echunk, fchunk);
         if (b->force_yc > 0) yc = b->force_yc;
         const int TX = b->TX;
         const int nshell = shell_count(6, g.nx, g.ny, g.nz);
@@ -494,7 +494,7 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             if (c->use_fused) {
                 // the same plan as the single-shot path, so a shot computes the same bits
                 // alone and in a batch
-                const FusedPlan P = fused_plan(d, g, TX, yc, c->fused_chunk, echunk);
+                const FusedPlan P = fused_plan(d, g, TX, yc, fchunk, echunk);
                 s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = P.nflux;
                 s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = 0;
                 s.nblk[grp == 1 ? B::C_EFUSED : B::C_VFUSED] = P.nfused;

@@ -111,6 +111,7 @@ struct fwi3d_ctx {
     struct Tiles {
         int physics, nx, ny, nz, tx, ychunk;
         int echunk;  // side-tile y chunk of the fused elastic / VTI kernel (0: by grid size)
+        int fchunk;  // its middle-tile y chunk (0: by grid size)
     };
     std::vector<Tiles> tuned;
     // time loops without host work between steps replay a captured graph of several steps

@@ -443,16 +443,18 @@ FusedPlan fused_plan(BShot& d, const Geom& g, int TX, int ychunk, int fchunk, in
 
 // Tile shape for a physics and grid: the autotuned one if fwi3d_autotune measured this
 // grid on the context, else the context's settings (fwi3d_tune). `echunk`: y chunk of
-// the fused kernel's side tiles.
-void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk, int& echunk) {
+// the fused kernel's side tiles, `fchunk` of its middle tiles.
+void tiles_for(const fwi3d_ctx* c, int physics, const Geom& g, int& TX, int& ychunk, int& echunk, int& fchunk) {
     TX = c->tune_tx;
     ychunk = c->tune_ychunk;
     echunk = c->fused_echunk;
+    fchunk = c->fused_chunk;
     for (const auto& t : c->tuned)
         if (t.physics == physics && t.nx == g.nx && t.ny == g.ny && t.nz == g.nz) {
             TX = t.tx;
             ychunk = t.ychunk;
             if (t.echunk > 0) echunk = t.echunk;
+            if (t.fchunk > 0) fchunk = t.fchunk;
         }
 }
 
@@ -597,8 +599,8 @@ void propagate_acoustic(fwi3d_ctx* c, bool adj, const float* d_model, const floa
     float* curr = F + n;
     float* next = F + 2 * n;
 
-    int TX, ychunk, echunk;
-    tiles_for(c, FWI3D_ACOUSTIC, g, TX, ychunk, echunk);
+    int TX, ychunk, echunk, fchunk;
+    tiles_for(c, FWI3D_ACOUSTIC, g, TX, ychunk, echunk, fchunk);
     AcuArgs A{};
     A.K = M; A.b = M + n; A.tz2 = tp.tz2; A.tx2 = tp.tx2; A.ty2 = tp.ty2; A.g = g;
     for (int f = 0; f < 6; f++) A.bc[f] = par->bc[f];
@@ -832,8 +834,8 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     };
     inject(prev, 0, 1);
 
-    int TX, ychunk, echunk;
-    tiles_for(c, vti ? FWI3D_VTI : FWI3D_ELASTIC, g, TX, ychunk, echunk);
+    int TX, ychunk, echunk, fchunk;
+    tiles_for(c, vti ? FWI3D_VTI : FWI3D_ELASTIC, g, TX, ychunk, echunk, fchunk);
     const int nblk = tile_grid_yc(A, g, TX, false, ychunk);
     EArgs AI = A;
     const int nint = tile_grid_yc(AI, g, TX, true, ychunk);
@@ -846,7 +848,7 @@ void propagate_elastic(fwi3d_ctx* c, bool vti, bool adj, const float* d_model, c
     const int nzslab = zslab_tiles(T1h.zs, g);
     const bool fused = c->use_fused;
     FusedPlan FP{};
-    if (fused) FP = fused_plan(T1h, g, TX, ychunk, c->fused_chunk, echunk);
+    if (fused) FP = fused_plan(T1h, g, TX, ychunk, fchunk, echunk);
     T1h.I = I; T1h.src = d_src; T1h.ninj_blk = nbi;
     T1h.E = E; T1h.rcv = d_rcv; T1h.next_blk = (E.npts + 127) / 128;
     const BShot* T1 = upload_shot1(c, T1h);
@@ -1289,9 +1291,10 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
     long long n0[fwi3d_ctx::kProfClasses];
     std::copy(c->prof_ms, c->prof_ms + fwi3d_ctx::kProfClasses, ms0);
     std::copy(c->prof_n, c->prof_n + fwi3d_ctx::kProfClasses, n0);
-    const int tx0 = c->tune_tx, yc0 = c->tune_ychunk, ec0 = c->fused_echunk;
+    const int tx0 = c->tune_tx, yc0 = c->tune_ychunk, ec0 = c->fused_echunk, fc0 = c->fused_chunk;
     auto restore = [&] {
-        c->tune_tx = tx0; c->tune_ychunk = yc0; c->fused_echunk = ec0; c->prof = prof0; c->prof_steps_only = steps_only0;
+        c->tune_tx = tx0; c->tune_ychunk = yc0; c->fused_echunk = ec0; c->fused_chunk = fc0;
+        c->prof = prof0; c->prof_steps_only = steps_only0;
         std::copy(ms0, ms0 + fwi3d_ctx::kProfClasses, c->prof_ms);
         std::copy(n0, n0 + fwi3d_ctx::kProfClasses, c->prof_n);
     };
@@ -1344,7 +1347,7 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
             return c->prof_ms[4] / steps;  // whole time steps (the kernels overlap)
         };
         run(4, 8);  // warm-up
-        int best_tx = 4, best_yc = 8, best_ec = 0;
+        int best_tx = 4, best_yc = 8, best_ec = 0, best_fc = 0;
         double best = 1e30;
         for (int tx : {4, 8})
             for (int yc : {4, 8, 16, 32}) {
@@ -1366,9 +1369,25 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
                 const double t = run(best_tx, best_yc);
                 if (t < 0.97 * best) { best = t; best_ec = ec; }
             }
+            // middle-tile y chunk: one y chunk fewer or more per tile column than the
+            // grid-size rule's (the middle tiles share the CUs with the edge tiles)
+            c->fused_echunk = best_ec > 0 ? best_ec : ec0;
+            if (fc0 <= 0) {
+                const int mid = fused_middle_tiles(ntz, ntx);
+                const int want = std::max(1, 512 / std::max(1, mid > 0 ? mid : ntz * ntx));
+                const int rule = std::max(8, (nyo + want - 1) / want);
+                for (int w : {want - 1, want + 1}) {
+                    if (w < 1) continue;
+                    const int fc = std::max(8, (nyo + w - 1) / w);
+                    if (fc == rule) continue;
+                    c->fused_chunk = fc;
+                    const double t = run(best_tx, best_yc);
+                    if (t < 0.97 * best) { best = t; best_fc = fc; }
+                }
+            }
         }
         restore();
-        c->tuned.push_back(fwi3d_ctx::Tiles{physics, nx, ny, nz, best_tx, best_yc, best_ec});
+        c->tuned.push_back(fwi3d_ctx::Tiles{physics, nx, ny, nz, best_tx, best_yc, best_ec, best_fc});
         if (tx_out) *tx_out = best_tx;
         if (ychunk_out) *ychunk_out = best_yc;
         m.release();
