@@ -492,9 +492,13 @@ int batch_add_impl(fwi3d_batch* b, int mode, int physics, const float* d_model, 
             s.nblk[grp == 1 ? B::C_ESHELL : B::C_VSHELL] = 0;  // the edges run in the slab class
             s.nblk[grp == 1 ? B::C_EZS : B::C_VZS] = zslab_tiles(d.zs, g) + 3 * d.nshell_blk;
             if (c->use_fused) {
-                // the same plan as the single-shot path, so a shot computes the same bits
-                // alone and in a batch
-                const FusedPlan P = fused_plan(d, g, TX, yc, fchunk, echunk);
+                // the single-shot path's plan with the context's fused y chunks, not the
+                // autotuned ones: those are timed for one shot alone (a VTI shot's edge tiles
+                // in 4 long chunks, fewer workgroups than CUs beside its middle tiles, made
+                // the 32-shot batch 1.3x slower). The y chunks do not change a result bit.
+                (void)fchunk;
+                (void)echunk;
+                const FusedPlan P = fused_plan(d, g, TX, yc, c->fused_chunk, c->fused_echunk);
                 s.nblk[grp == 1 ? B::C_EFLUX : B::C_VFLUX] = P.nflux;
                 s.nblk[grp == 1 ? B::C_EINT : B::C_VINT] = 0;
                 s.nblk[grp == 1 ? B::C_EFUSED : B::C_VFUSED] = P.nfused;

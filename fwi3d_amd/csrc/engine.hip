@@ -1355,13 +1355,15 @@ int fwi3d_autotune(fwi3d_ctx* c, int physics, int nx, int ny, int nz, int steps,
                 if (t < best) { best = t; best_tx = tx; best_yc = yc; }
             }
         if (physics != FWI3D_ACOUSTIC && c->use_fused && ny > 12) {
-            // y chunk of the fused kernel's side tiles: about 1, 1.5, 2 or 2.5 side
-            // workgroups per CU (the edge tiles run one workgroup per CU). A chunk other
+            // y chunk of the fused kernel's side tiles: about 0.6, 0.75, 1, 1.5, 2 or 2.5
+            // side workgroups per CU (the edge tiles run one workgroup per CU; beside the
+            // middle tiles, fewer edge workgroups than CUs can be fastest: 256^3 414
+            // against 451 us per step with 160 side workgroups). A chunk other
             // than the grid-size rule's must be 3 % faster: the timings of 10-step runs
             // vary by a few per cent, and a noisy pick cost a batch of three grids 10 %.
             const int ntz = fused_tile_count(nz, FZ), ntx = fused_tile_count(nx, FW), nyo = ny - 12;
-            int tried[4], ntried = 0;
-            for (int target : {256, 384, 512, 640}) {
+            int tried[6], ntried = 0;
+            for (int target : {160, 192, 256, 384, 512, 640}) {
                 const int ec = fused_side_chunk(ntz, ntx, nyo, target);
                 if (std::find(tried, tried + ntried, ec) != tried + ntried) continue;
                 tried[ntried++] = ec;

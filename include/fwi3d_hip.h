@@ -159,8 +159,8 @@ enum fwi3d_tune_param {
 int fwi3d_tune(fwi3d_ctx *ctx, int param, int value);
 /* Autotuner (no counterpart in the reference): times `steps` time steps of the given
  * physics and grid on a synthetic model for every tile shape (TX 4 or 8, y chunk 4, 8,
- * 16 or 32, and for the fused elastic / VTI step the y chunk of its side tiles, about 1,
- * 1.5, 2 or 2.5 side workgroups per CU, and of its middle tiles, one chunk fewer or more
+ * 16 or 32, and for the fused elastic / VTI step the y chunk of its side tiles, about
+ * 0.6 to 2.5 side workgroups per CU, and of its middle tiles, one chunk fewer or more
  * per tile column than the grid-size rule's; a chunk other than the rule's is kept when
  * it is at least 3 % faster) with HIP events, keeps the fastest for that
  * physics and grid on the context
